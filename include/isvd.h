@@ -1,0 +1,220 @@
+/*
+ * isvd.h -- public C ABI of the MI355X (gfx950) iSVD library (libisvd_mi355x.so).
+ *
+ * Drop-in replacement for the stage and driver entry points of emfomy/isvd v1.3.2
+ * (isvd_core + isvd_la_* + isvd_gpu_*): same function names, argument order and meaning,
+ * same column-major layouts, same selector strings.  Each declaration cites the reference
+ * declaration it replaces (paths relative to the reference repository).
+ *
+ * Differences a caller can observe:
+ *   - One process, one GPU (device 0).  isvd_MpiComm is an integer stand-in that is ignored;
+ *     isvd_createParam always describes P = 1 (the whole matrix on this GPU).
+ *   - Every matrix argument may be a host pointer or a device (HBM) pointer.  Device pointers
+ *     are used in place (no PCIe traffic); host pointers are staged through HBM.
+ *   - Failures that the reference turns into assert() print to stderr, record a message for
+ *     isvd_GetLastError() and return; set ISVD_ASSERT_ABORT=1 to abort like the reference.
+ *     Unknown selector strings / characters print and return silently, as the reference does.
+ *   - isvd_createParam does not reject m > n (the reference asserts m <= n, param.c:43).
+ *   - Omega is drawn on the GPU from `seed` (Philox-4x32-10 + Box-Muller); the stream differs
+ *     from MKL's SFMT19937 stream of the reference.  The *Omega extensions below take a
+ *     caller-supplied Omega instead.
+ *   - Tall-skinny-QR postprocessing returns the mathematically correct U and V for k < l
+ *     (the reference swaps the SVD factors of R there; identical when k == l).
+ *   - Dimension limit: l = k + p <= 128 for the orthogonalization / integration /
+ *     postprocessing kernels.
+ */
+#ifndef ISVD_MI355X_ISVD_H_
+#define ISVD_MI355X_ISVD_H_
+
+#include <stddef.h>
+#include <stdint.h>
+#include <stdio.h>
+
+#if defined(__cplusplus)
+extern "C" {
+#endif
+
+/* ---- types: src/include/c/isvd/def.h:45-95 ---- */
+#if !defined(ISVD_USE_ILP64)
+typedef int32_t isvd_int_t;
+#else
+typedef int64_t isvd_int_t;
+#endif
+typedef float isvd_s_val_t;
+typedef double isvd_d_val_t;
+typedef int mpi_int_t;
+typedef int isvd_MpiComm; /* replaces MPI_Comm (def.h:95); ignored, P = 1 */
+
+#define ISVD_COMM_WORLD 0
+
+/* ---- parameters: src/include/c/isvd/core/param.h:27-92 (same fields, same order) ---- */
+typedef struct {
+  const isvd_MpiComm mpi_comm;
+  const mpi_int_t mpi_size;
+  const mpi_int_t mpi_root;
+  const mpi_int_t mpi_rank;
+  const isvd_int_t nrow;
+  const isvd_int_t ncol;
+  const isvd_int_t nrow_proc;
+  const isvd_int_t ncol_proc;
+  const isvd_int_t nrow_each;
+  const isvd_int_t ncol_each;
+  const isvd_int_t nrow_total;
+  const isvd_int_t ncol_total;
+  const isvd_int_t rowidxbegin;
+  const isvd_int_t rowidxend;
+  const isvd_int_t colidxbegin;
+  const isvd_int_t colidxend;
+  const isvd_int_t rank;
+  const isvd_int_t over_rank;
+  const isvd_int_t dim_sketch;
+  const isvd_int_t dim_sketch_total;
+  const isvd_int_t num_sketch;
+} isvd_Param;
+
+/* replaces isvd_createParam (src/include/c/isvd/core/param.h:100-103, param.c:14-83) */
+isvd_Param isvd_createParam(const isvd_int_t nrow, const isvd_int_t ncol, const isvd_int_t rank,
+                            const isvd_int_t over_rank, const isvd_int_t num_sketch,
+                            const mpi_int_t mpi_root, const isvd_MpiComm mpi_comm);
+
+/* ---- environment: src/include/c/isvd/core/env.h:30-55, src/include/c/isvd/gpu/env.h:19-25 ---- */
+extern size_t isvd_gpu_memory_limit; /* gpu/env.h:19; bytes of HBM the library may use (0 = all) */
+void isvd_Init_cpu(int *argcp, char ***argvp, const isvd_MpiComm mpi_comm);
+void isvd_Finalize_cpu(void);
+void isvd_PrintEnvironment_cpu(const isvd_MpiComm comm);
+void isvd_Init_gpu(int *argcp, char ***argvp, const isvd_MpiComm mpi_comm);
+void isvd_Finalize_gpu(void);
+void isvd_PrintEnvironment_gpu(const isvd_MpiComm comm);
+
+static inline void isvd_Init(int *argcp, char ***argvp, const isvd_MpiComm mpi_comm) {
+  isvd_Init_cpu(argcp, argvp, mpi_comm);
+  isvd_Init_gpu(argcp, argvp, mpi_comm);
+}
+static inline void isvd_Finalize(void) {
+  isvd_Finalize_cpu();
+  isvd_Finalize_gpu();
+}
+static inline void isvd_PrintEnvironment(const isvd_MpiComm mpi_comm) {
+  isvd_PrintEnvironment_cpu(mpi_comm);
+  isvd_PrintEnvironment_gpu(mpi_comm);
+}
+
+/* ---- host memory: src/include/c/isvd/util/memory.h:19-25 (64-byte aligned host memory) ---- */
+isvd_s_val_t *isvd_sMalloc(const size_t num);
+isvd_d_val_t *isvd_dMalloc(const size_t num);
+void isvd_Free(void *ptr);
+
+/* ---- error state (no reference counterpart: the reference asserts) ---- */
+const char *isvd_GetLastError(void); /* "" when the last call succeeded */
+int isvd_GetErrorCount(void);        /* failures since load */
+
+/* =========================================================================================
+ * Driver: replaces isvd_?Isvd (src/include/c/isvd/core/@x@_driver.h:21-29,
+ * src/lib/libisvd/core/driver/@x@_isvd.c:106-282).  Selectors (case-insensitive):
+ *   alg_s "GP" | "GP_gpu";  alg_o "TS" | "GR";  alg_i "KN" | "WY";
+ *   alg_p "TS" | "GR" | "TS_gpu" | "GR_gpu".   All run on the GPU.
+ * time[4] receives per-stage seconds (GPU event time, stream-ordered).
+ * ========================================================================================= */
+#define ISVD_DECLARE_DRIVER(x, T)                                                              \
+  void isvd_##x##Isvd(const isvd_Param param, const char *alg_s, const char *alg_o,           \
+                      const char *alg_i, const char *alg_p, const T *argv[4],                 \
+                      const isvd_int_t argc[4], T *retv[4], const isvd_int_t retc[4],         \
+                      double time[4], FILE *stream, const char dista, const char ordera,      \
+                      const T *a, const isvd_int_t lda, T *s, T *ut, const isvd_int_t ldut,   \
+                      T *vt, const isvd_int_t ldvt, const isvd_int_t seed,                    \
+                      const mpi_int_t ut_root, const mpi_int_t vt_root);
+ISVD_DECLARE_DRIVER(s, isvd_s_val_t)
+ISVD_DECLARE_DRIVER(d, isvd_d_val_t)
+
+/* Extension of isvd_?Isvd: Omega (n x N*l, stored row-major, i.e. omegat = Omega^T col-major
+ * with leading dimension ldomegat >= N*l) is supplied by the caller instead of drawn from a
+ * seed.  Used for parity runs where both sides consume the same host-generated Omega. */
+#define ISVD_DECLARE_DRIVER_OMEGA(x, T)                                                        \
+  void isvd_##x##IsvdOmega(const isvd_Param param, const char *alg_s, const char *alg_o,      \
+                           const char *alg_i, const char *alg_p, const T *argv[4],            \
+                           const isvd_int_t argc[4], T *retv[4], const isvd_int_t retc[4],    \
+                           double time[4], FILE *stream, const char dista, const char ordera, \
+                           const T *a, const isvd_int_t lda, const T *omegat,                 \
+                           const isvd_int_t ldomegat, T *s, T *ut, const isvd_int_t ldut,     \
+                           T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,             \
+                           const mpi_int_t vt_root);
+ISVD_DECLARE_DRIVER_OMEGA(s, isvd_s_val_t)
+ISVD_DECLARE_DRIVER_OMEGA(d, isvd_d_val_t)
+
+/* =========================================================================================
+ * Stages: replace src/include/c/isvd/core/@x@_stage.h:20-77 (and the GPU twins of
+ * src/include/c/isvd/gpu/@x@_stage.h:28-80, which here are the same implementation).
+ * ========================================================================================= */
+#define ISVD_DECLARE_STAGES(x, T)                                                              \
+  /* @x@_stage.h:21-25 / @x@_sketch_gaussian_projection.c:46-84 */                            \
+  void isvd_##x##SketchGaussianProjection(                                                     \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, T *yst, const isvd_int_t ldyst, const isvd_int_t seed,            \
+      const mpi_int_t mpi_root);                                                              \
+  /* gpu/@x@_stage.h:28-33 */                                                                 \
+  void isvd_##x##SketchGaussianProjection_gpu(                                                 \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, T *yst, const isvd_int_t ldyst, const isvd_int_t seed,            \
+      const mpi_int_t mpi_root);                                                              \
+  /* extension: caller-supplied Omega (omegat = Omega^T, Nl x n col-major, ld >= Nl) */        \
+  void isvd_##x##SketchGaussianProjectionOmega(                                                \
+      const isvd_Param param, const char dista, const char ordera, const T *a,                \
+      const isvd_int_t lda, const T *omegat, const isvd_int_t ldomegat, T *yst,               \
+      const isvd_int_t ldyst);                                                                \
+  /* @x@_stage.h:28-31 / @x@_orthogonalize_tall_skinny_qr.c:36-212 */                         \
+  void isvd_##x##OrthogonalizeTallSkinnyQr(const isvd_Param param, const T *argv,             \
+                                           const isvd_int_t argc, T *retv,                    \
+                                           const isvd_int_t retc, T *yst,                     \
+                                           const isvd_int_t ldyst);                           \
+  /* @x@_stage.h:33-36 / @x@_orthogonalize_gramian.c:33-107 */                                \
+  void isvd_##x##OrthogonalizeGramian(const isvd_Param param, const T *argv,                  \
+                                      const isvd_int_t argc, T *retv, const isvd_int_t retc,  \
+                                      T *yst, const isvd_int_t ldyst);                        \
+  /* @x@_stage.h:39-42 / @x@_integrate_kolmogorov_nagumo.c:42-277 */                          \
+  void isvd_##x##IntegrateKolmogorovNagumo(const isvd_Param param, const T *argv,             \
+                                           const isvd_int_t argc, T *retv,                    \
+                                           const isvd_int_t retc, const T *yst,               \
+                                           const isvd_int_t ldyst, T *qt,                     \
+                                           const isvd_int_t ldqt);                            \
+  /* @x@_stage.h:44-47 / @x@_integrate_wen_yin.c:56-386 */                                    \
+  void isvd_##x##IntegrateWenYin(const isvd_Param param, const T *argv, const isvd_int_t argc, \
+                                 T *retv, const isvd_int_t retc, const T *yst,                \
+                                 const isvd_int_t ldyst, T *qt, const isvd_int_t ldqt);       \
+  /* @x@_stage.h:55-62 / @x@_postprocess_tall_skinny_qr.c:67-309 */                           \
+  void isvd_##x##PostprocessTallSkinnyQr(                                                      \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root);                                                               \
+  /* @x@_stage.h:64-71 / @x@_postprocess_gramian.c:64-193 */                                  \
+  void isvd_##x##PostprocessGramian(                                                           \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root);                                                               \
+  /* gpu/@x@_stage.h:42-48 */                                                                 \
+  void isvd_##x##PostprocessTallSkinnyQr_gpu(                                                  \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root);                                                               \
+  /* gpu/@x@_stage.h:57-63 */                                                                 \
+  void isvd_##x##PostprocessGramian_gpu(                                                       \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root);
+ISVD_DECLARE_STAGES(s, isvd_s_val_t)
+ISVD_DECLARE_STAGES(d, isvd_d_val_t)
+
+#if defined(__cplusplus)
+}
+#endif
+
+#endif /* ISVD_MI355X_ISVD_H_ */

@@ -1,0 +1,692 @@
+// C ABI of libisvd_mi355x.so: the reference's public entry points (include/isvd.h) over the
+// device stages.  Host pointers are staged through HBM; device pointers are used in place.
+#include <hip/hip_runtime.h>
+
+#include <cctype>
+#include <chrono>
+#include <cstdarg>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "../../include/isvd.h"
+#include "la.hpp"
+#include "stages.hpp"
+
+size_t isvd_gpu_memory_limit = 0;
+
+namespace isvd {
+
+// ---------------------------------------------------------------------------------------------
+// error state
+// ---------------------------------------------------------------------------------------------
+static thread_local std::string g_last_error;
+static int g_error_count = 0;
+
+void set_last_error(const std::string &msg) { g_last_error = msg; }
+
+void fail(const std::string &msg) {
+  std::fprintf(stderr, "isvd: %s\n", msg.c_str());
+  std::fflush(stderr);
+  g_last_error = msg;
+  ++g_error_count;
+  const char *ab = std::getenv("ISVD_ASSERT_ABORT");
+  if (ab && ab[0] == '1') std::abort();
+  throw Error{msg};
+}
+
+// ---------------------------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------------------------
+static Ctx g_ctx;
+static bool g_init = false;
+static std::mutex g_mutex;
+
+Ctx &ctx() { return g_ctx; }
+
+void ensure_init() {
+  if (g_init) return;
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev < 1) {
+    (void)hipGetLastError();
+    fail("no HIP device available (the iSVD library runs on an MI355X GPU)");
+  }
+  HIP_CHECK(hipSetDevice(0));
+  g_ctx.device = 0;
+  HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  g_init = true;
+}
+
+void sync() { HIP_CHECK(hipStreamSynchronize(g_ctx.stream)); }
+
+bool is_device_pointer(const void *ptr) {
+  if (!ptr) return false;
+  hipPointerAttribute_t attr;
+  std::memset(&attr, 0, sizeof(attr));
+  hipError_t e = hipPointerGetAttributes(&attr, ptr);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice || attr.type == hipMemoryTypeManaged;
+}
+
+// A matrix argument: uses the caller's device memory in place, or stages host memory.
+template <typename T>
+struct Arg {
+  T *host = nullptr;
+  T *dev = nullptr;
+  i64 count = 0;
+  bool out = false;
+  DBuf<T> buf;
+  Arg(const T *p, i64 n, bool in, bool out_) : count(n), out(out_) {
+    if (!p || n <= 0) return;
+    if (is_device_pointer(p)) {
+      dev = const_cast<T *>(p);
+    } else {
+      host = const_cast<T *>(p);
+      buf.alloc(n);
+      dev = buf.get();
+      if (in)
+        HIP_CHECK(hipMemcpyAsync(dev, host, sizeof(T) * n, hipMemcpyHostToDevice, ctx().stream));
+    }
+  }
+  void finish() {
+    if (host && out)
+      HIP_CHECK(hipMemcpyAsync(host, dev, sizeof(T) * count, hipMemcpyDeviceToHost, ctx().stream));
+  }
+};
+
+// elements spanned by an m x n column-major matrix with leading dimension ld
+inline i64 span(i64 m, i64 n, i64 ld) { return (m <= 0 || n <= 0) ? 0 : (n - 1) * ld + m; }
+
+// ---------------------------------------------------------------------------------------------
+// argument parsing: isvd_arg2char / isvd_arg2str (src/lib/libisvd/util/arg.h:22-79)
+// ---------------------------------------------------------------------------------------------
+static char arg2char(const char *name, char arg, const char *opts) {
+  for (const char *o = opts; *o; ++o)
+    if (std::toupper((unsigned char)arg) == *o) return *o;
+  std::fprintf(stderr, "%s ('%c') must be one of the following characters: ", name, arg);
+  for (const char *o = opts; *o; ++o) std::fprintf(stderr, o == opts ? "'%c'" : ", '%c'", *o);
+  std::fprintf(stderr, ".\n");
+  return 0;
+}
+
+static const char *arg2str(const char *name, const char *arg, const char *const *opts, int nopts) {
+  if (!arg) arg = "";
+  for (int i = 0; i < nopts; ++i) {
+    const char *o = opts[i];
+    if (std::strlen(o) != std::strlen(arg)) continue;
+    bool same = true;
+    for (size_t q = 0; o[q]; ++q)
+      if (std::toupper((unsigned char)o[q]) != std::toupper((unsigned char)arg[q])) same = false;
+    if (same) return o;
+  }
+  std::fprintf(stderr, "%s ('%s') must be one of the following strings: ", name, arg);
+  for (int i = 0; i < nopts; ++i) std::fprintf(stderr, i == 0 ? "\"%s\"" : ", \"%s\"", opts[i]);
+  std::fprintf(stderr, ".\n");
+  return nullptr;
+}
+
+static Dims dims_of(const isvd_Param &p) {
+  Dims d;
+  d.m = p.nrow; d.n = p.ncol; d.k = p.rank; d.l = p.dim_sketch; d.N = p.num_sketch;
+  d.Nl = p.dim_sketch_total;
+  return d;
+}
+
+static void check_param(const isvd_Param &p) {
+  ISVD_CHECK(p.nrow > 0 && p.ncol > 0 && p.rank > 0 && p.over_rank >= 0 && p.num_sketch > 0,
+             "invalid parameters (m=%lld n=%lld k=%lld p=%lld N=%lld)", (i64)p.nrow,
+             (i64)p.ncol, (i64)p.rank, (i64)p.over_rank, (i64)p.num_sketch);
+  ISVD_CHECK(p.dim_sketch == p.rank + p.over_rank &&
+                 p.dim_sketch_total == p.dim_sketch * p.num_sketch,
+             "inconsistent parameters (l=%lld, Nl=%lld)", (i64)p.dim_sketch,
+             (i64)p.dim_sketch_total);
+  ISVD_CHECK(p.mpi_size == 1, "this library runs on one GPU (mpi_size must be 1)");
+}
+
+// Runs `body` under the library lock; converts failures into an error return.
+template <typename F>
+static bool guarded(F &&body) {
+  std::lock_guard<std::mutex> lock(g_mutex);
+  g_last_error.clear();
+  try {
+    body();
+    return true;
+  } catch (const Error &) {
+    if (g_init) (void)hipStreamSynchronize(g_ctx.stream);
+    return false;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// stage entry points (templated, wrapped by the extern "C" functions below)
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+static void sketch_impl(const isvd_Param &param, char dista, char ordera, const T *a, i64 lda,
+                        const T *omegat, i64 ldo, T *yst, i64 ldyst, bool use_seed,
+                        unsigned long long seed) {
+  const char dista_ = arg2char("DISTA", dista, "CR");
+  const char ordera_ = arg2char("ORDERA", ordera, "CR");
+  if (!dista_ || !ordera_) return;
+  check_param(param);
+  const Dims d = dims_of(param);
+  // @x@_sketch.h:50-55 (block-col) / :115-120 (block-row)
+  ISVD_CHECK(lda >= (ordera_ == 'C' ? d.m : d.n), "lda (%lld) too small", lda);
+  if (dista_ == 'C')
+    ISVD_CHECK(ldyst == d.Nl, "ldyst (%lld) must equal Nl (%lld)", ldyst, d.Nl);
+  else
+    ISVD_CHECK(ldyst >= d.Nl, "ldyst (%lld) must be >= Nl (%lld)", ldyst, d.Nl);
+  ensure_init();
+  const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
+  Arg<T> A(a, a_span, true, false);
+  Arg<T> Y(yst, span(d.Nl, d.m, ldyst), false, true);
+  DBuf<T> om;
+  const T *omd;
+  if (use_seed) {
+    om.alloc(d.Nl * d.n);
+    gaussian_omega<T>(d.Nl, d.n, om.get(), d.Nl, seed);
+    omd = om.get();
+    ldo = d.Nl;
+  } else {
+    ISVD_CHECK(ldo >= d.Nl, "ldomegat (%lld) must be >= Nl (%lld)", ldo, d.Nl);
+    Arg<T> O(omegat, span(d.Nl, d.n, ldo), true, false);
+    stage_sketch<T>(d, ordera_, A.dev, lda, O.dev, ldo, Y.dev, ldyst);
+    Y.finish();
+    sync();
+    return;
+  }
+  stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, Y.dev, ldyst);
+  Y.finish();
+  sync();
+}
+
+template <typename T>
+static void orth_impl(const isvd_Param &param, i64 argc, T *yst, i64 ldyst, bool tsqr) {
+  if (argc < 0) return;
+  check_param(param);
+  const Dims d = dims_of(param);
+  ISVD_CHECK(ldyst >= d.Nl, "ldyst (%lld) must be >= Nl (%lld)", ldyst, d.Nl);
+  ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  if (tsqr) ISVD_CHECK(d.m >= d.l, "tall-skinny QR needs m >= l");
+  ensure_init();
+  Arg<T> Y(yst, span(d.Nl, d.m, ldyst), true, true);
+  if (tsqr)
+    stage_orth_tsqr<T>(d, Y.dev, ldyst);
+  else
+    stage_orth_gramian<T>(d, Y.dev, ldyst);
+  Y.finish();
+  sync();
+}
+
+template <typename T>
+static void kn_impl(const isvd_Param &param, const T *argv, i64 argc, T *retv, i64 retc,
+                    const T *yst, i64 ldyst, T *qt, i64 ldqt) {
+  KnArgs ka;
+  // query (@x@_integrate_kolmogorov_nagumo.c:56-61)
+  if (argc < 0) {
+    if (retc > 0) retv[0] = (T)ka.maxit;
+    if (retc > 1) retv[1] = (T)ka.tol;
+    return;
+  }
+  if (argc > 0) ISVD_CHECK(argv != nullptr, "argv is NULL");
+  if (retc > 0) ISVD_CHECK(retv != nullptr, "retv is NULL");
+  if (argc > 0) ka.maxit = (i64)argv[0];
+  if (argc > 1) ka.tol = (double)argv[1];
+  check_param(param);
+  const Dims d = dims_of(param);
+  ISVD_CHECK(ldyst >= d.Nl, "ldyst (%lld) must be >= Nl", ldyst);
+  ISVD_CHECK(ldqt >= d.l, "ldqt (%lld) must be >= l", ldqt);
+  ISVD_CHECK(ka.maxit >= 0 && ka.tol >= 0, "maxit and tol must be >= 0");
+  ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  ensure_init();
+  Arg<T> Y(yst, span(d.Nl, d.m, ldyst), true, false);
+  Arg<T> Q(qt, span(d.l, d.m, ldqt), false, true);
+  i64 it = 0;
+  double err = 0;
+  stage_integrate_kn<T>(d, ka, Y.dev, ldyst, Q.dev, ldqt, &it, &err);
+  Q.finish();
+  sync();
+  if (retc > 0) retv[0] = (T)it;
+  if (retc > 1) retv[1] = (T)err;
+}
+
+template <typename T>
+static void wy_impl(const isvd_Param &param, const T *argv, i64 argc, T *retv, i64 retc,
+                    const T *yst, i64 ldyst, T *qt, i64 ldqt) {
+  WyArgs wa;
+  const double defaults[9] = {(double)wa.maxit, wa.tol, wa.tau0, wa.taumax, wa.taumin,
+                              (double)wa.taumaxit, wa.beta, wa.sigma, wa.eta};
+  if (argc < 0) {  // query (@x@_integrate_wen_yin.c:70-82)
+    for (i64 i = 0; i < retc && i < 9; ++i) retv[i] = (T)defaults[i];
+    return;
+  }
+  if (argc > 0) ISVD_CHECK(argv != nullptr, "argv is NULL");
+  if (retc > 0) ISVD_CHECK(retv != nullptr, "retv is NULL");
+  double v[9];
+  for (int i = 0; i < 9; ++i) v[i] = (argc > i) ? (double)argv[i] : defaults[i];
+  wa.maxit = (i64)v[0]; wa.tol = v[1]; wa.tau0 = v[2]; wa.taumax = v[3]; wa.taumin = v[4];
+  wa.taumaxit = (i64)v[5]; wa.beta = v[6]; wa.sigma = v[7]; wa.eta = v[8];
+  check_param(param);
+  const Dims d = dims_of(param);
+  ISVD_CHECK(ldyst >= d.Nl, "ldyst (%lld) must be >= Nl", ldyst);
+  ISVD_CHECK(ldqt >= d.l, "ldqt (%lld) must be >= l", ldqt);
+  // (@x@_integrate_wen_yin.c:103-113)
+  ISVD_CHECK(wa.maxit >= 0 && wa.tol >= 0 && wa.tau0 >= 0 && wa.taumax >= 0 && wa.taumin >= 0 &&
+                 wa.taumaxit >= 0,
+             "WY arguments must be >= 0");
+  ISVD_CHECK(wa.beta > 0 && wa.beta < 1 && wa.sigma > 0 && wa.sigma < 1 && wa.eta > 0 &&
+                 wa.eta < 1,
+             "WY beta, sigma, eta must lie in (0, 1)");
+  ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  ensure_init();
+  Arg<T> Y(yst, span(d.Nl, d.m, ldyst), true, false);
+  Arg<T> Q(qt, span(d.l, d.m, ldqt), false, true);
+  i64 it = 0;
+  double mu = 0;
+  stage_integrate_wy<T>(d, wa, Y.dev, ldyst, Q.dev, ldqt, &it, &mu);
+  Q.finish();
+  sync();
+  if (retc > 0) retv[0] = (T)it;
+  if (retc > 1) retv[1] = (T)mu;
+}
+
+template <typename T>
+static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera, const T *a,
+                      i64 lda, const T *qt, i64 ldqt, T *s, T *ut, i64 ldut, T *vt, i64 ldvt,
+                      int ut_root, int vt_root, bool tsqr) {
+  if (argc < 0) return;
+  const char dista_ = arg2char("DISTA", dista, "CR");
+  const char ordera_ = arg2char("ORDERA", ordera, "CR");
+  if (!dista_ || !ordera_) return;
+  check_param(param);
+  const Dims d = dims_of(param);
+  // @x@_postprocess_gramian.c:99-119 / @x@_postprocess.h:44-54, :121-131
+  if (ut_root >= 0) ISVD_CHECK(ldut == d.l, "ldut (%lld) must equal l (%lld)", ldut, d.l);
+  else if (ut_root == -1) ISVD_CHECK(ldut >= d.l, "ldut (%lld) must be >= l", ldut);
+  if (vt_root >= 0) ISVD_CHECK(ldvt == d.l, "ldvt (%lld) must equal l (%lld)", ldvt, d.l);
+  else if (vt_root == -1) ISVD_CHECK(ldvt >= d.l, "ldvt (%lld) must be >= l", ldvt);
+  ISVD_CHECK(lda >= (ordera_ == 'C' ? d.m : d.n), "lda (%lld) too small", lda);
+  ISVD_CHECK(ldqt >= d.l, "ldqt (%lld) must be >= l", ldqt);
+  ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  ISVD_CHECK(s != nullptr, "s is NULL");
+  const bool want_u = ut_root >= -1, want_v = vt_root >= -1;
+  if (want_u) ISVD_CHECK(ut != nullptr, "ut is NULL");
+  if (want_v) ISVD_CHECK(vt != nullptr, "vt is NULL");
+  ensure_init();
+  const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
+  Arg<T> A(a, a_span, true, false);
+  Arg<T> Q(qt, span(d.l, d.m, ldqt), true, false);
+  Arg<T> S(s, d.l, false, true);
+  Arg<T> U(want_u ? ut : nullptr, span(d.k, d.m, ldut), false, true);
+  Arg<T> V(want_v ? vt : nullptr, span(d.k, d.n, ldvt), false, true);
+  if (tsqr)
+    stage_post_tsqr<T>(d, ordera_, A.dev, lda, Q.dev, ldqt, S.dev, U.dev, ldut, V.dev, ldvt,
+                       want_u, want_v);
+  else
+    stage_post_gramian<T>(d, ordera_, A.dev, lda, Q.dev, ldqt, S.dev, U.dev, ldut, V.dev, ldvt,
+                          want_u, want_v);
+  S.finish();
+  U.finish();
+  V.finish();
+  sync();
+}
+
+// ---------------------------------------------------------------------------------------------
+// driver (src/lib/libisvd/core/driver/@x@_isvd.c:106-282)
+// ---------------------------------------------------------------------------------------------
+static const char *const kOptsS[] = {"GP", "GP_gpu"};
+static const char *const kOptsO[] = {"TS", "GR"};
+static const char *const kOptsI[] = {"KN", "WY"};
+static const char *const kOptsP[] = {"TS", "GR", "TS_gpu", "GR_gpu"};
+
+static const char *alg_name(char kind, const char *alg, bool dbl) {
+  static thread_local std::string s;
+  const char *prec = dbl ? "Double" : "Single";
+  std::string name;
+  if (kind == 'S') name = "Gaussian Projection Sketching";
+  else if (kind == 'O') name = (alg[0] == 'T') ? "Tall-Skinny QR Orthogonalization" : "Gramian Orthogonalization";
+  else if (kind == 'I') name = (alg[0] == 'K') ? "Kolmogorov-Nagumo Integration" : "Wen-Yin Integration";
+  else name = (alg[0] == 'T') ? "Tall-Skinny QR Postprocessing" : "Gramian Postprocessing";
+  s = std::string(prec) + " Precision GPU (MI355X) " + name;
+  return s.c_str();
+}
+
+template <typename T>
+static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *alg_o,
+                      const char *alg_i, const char *alg_p, const T *argv[4], const isvd_int_t argc[4],
+                      T *retv[4], const isvd_int_t retc[4], double time[4], FILE *stream,
+                      char dista, char ordera, const T *a, i64 lda, const T *omegat, i64 ldo,
+                      T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool use_seed,
+                      unsigned long long seed, int ut_root, int vt_root) {
+  const char *as = arg2str("ALG_S", alg_s, kOptsS, 2);
+  const char *ao = arg2str("ALG_O", alg_o, kOptsO, 2);
+  const char *ai = arg2str("ALG_I", alg_i, kOptsI, 2);
+  const char *ap = arg2str("ALG_P", alg_p, kOptsP, 4);
+  if (!as || !ao || !ai || !ap) return;
+  const T *argv_i = argv ? argv[2] : nullptr;
+  const i64 argc_s = argv ? argc[0] : 0, argc_o = argv ? argc[1] : 0, argc_i = argv ? argc[2] : 0,
+            argc_p = argv ? argc[3] : 0;
+  T *retv_i = retv ? retv[2] : nullptr;
+  const i64 retc_i = retv ? retc[2] : 0;
+  if (argc_i > 0) ISVD_CHECK(argv_i != nullptr, "argv[2] is NULL");
+  if (retc_i > 0) ISVD_CHECK(retv_i != nullptr, "retv[2] is NULL");
+  const bool kn = (ai[0] == 'K');
+  // query mode (@x@_isvd.c:199-218)
+  if (argc_s < 0 || argc_o < 0 || argc_i < 0 || argc_p < 0) {
+    if (argc_i < 0) {
+      if (kn) kn_impl<T>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
+      else wy_impl<T>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
+    }
+    return;
+  }
+  const char dista_ = arg2char("DISTA", dista, "CR");
+  const char ordera_ = arg2char("ORDERA", ordera, "CR");
+  if (!dista_ || !ordera_) return;
+  check_param(param);
+  const Dims d = dims_of(param);
+  const bool dbl = sizeof(T) == 8;
+  if (stream) {
+    std::fprintf(stream, "Using %s.\n", alg_name('S', as, dbl));
+    std::fprintf(stream, "Using %s.\n", alg_name('O', ao, dbl));
+    std::fprintf(stream, "Using %s.\n", alg_name('I', ai, dbl));
+    std::fprintf(stream, "Using %s.\n\n", alg_name('P', ap, dbl));
+  }
+  // argument checks of every stage, before any device work
+  ISVD_CHECK(lda >= (ordera_ == 'C' ? d.m : d.n), "lda (%lld) too small", lda);
+  ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  const bool want_u = ut_root >= -1, want_v = vt_root >= -1;
+  if (ut_root >= 0) ISVD_CHECK(ldut == d.l, "ldut (%lld) must equal l (%lld)", ldut, d.l);
+  else if (ut_root == -1) ISVD_CHECK(ldut >= d.l, "ldut (%lld) must be >= l", ldut);
+  if (vt_root >= 0) ISVD_CHECK(ldvt == d.l, "ldvt (%lld) must equal l (%lld)", ldvt, d.l);
+  else if (vt_root == -1) ISVD_CHECK(ldvt >= d.l, "ldvt (%lld) must be >= l", ldvt);
+  ISVD_CHECK(s != nullptr, "s is NULL");
+  if (ao[0] == 'T') ISVD_CHECK(d.m >= d.l, "tall-skinny QR needs m >= l");
+  if (ap[0] == 'T') ISVD_CHECK(d.n >= d.l, "tall-skinny QR postprocessing needs n >= l");
+  if (!use_seed) ISVD_CHECK(ldo >= d.Nl, "ldomegat (%lld) must be >= Nl", ldo);
+  KnArgs ka;
+  WyArgs wa;
+  if (argc_i > 0) {
+    if (kn) {
+      ka.maxit = (i64)argv_i[0];
+      if (argc_i > 1) ka.tol = (double)argv_i[1];
+    } else {
+      double *f[9] = {nullptr, &wa.tol, &wa.tau0, &wa.taumax, &wa.taumin, nullptr, &wa.beta,
+                      &wa.sigma, &wa.eta};
+      for (i64 i = 0; i < argc_i && i < 9; ++i) {
+        if (i == 0) wa.maxit = (i64)argv_i[0];
+        else if (i == 5) wa.taumaxit = (i64)argv_i[5];
+        else *f[i] = (double)argv_i[i];
+      }
+    }
+  }
+  (void)argc_s; (void)argc_o; (void)argc_p;
+  ensure_init();
+  const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
+  Arg<T> A(a, a_span, true, false);
+  Arg<T> S(s, d.l, false, true);
+  Arg<T> U(want_u ? ut : nullptr, span(d.k, d.m, ldut), false, true);
+  Arg<T> V(want_v ? vt : nullptr, span(d.k, d.n, ldvt), false, true);
+  DBuf<T> yst(d.Nl * d.m), qt(d.l * d.m), om;
+  hipEvent_t ev[5];
+  for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
+  auto stage_msg = [&](const char *msg) {
+    if (stream) { std::fprintf(stream, "%s", msg); std::fflush(stream); }
+  };
+  i64 it = 0;
+  double err = 0;
+  // sketching (@x@_isvd.c:243-247)
+  stage_msg("Sketching ..................... ");
+  HIP_CHECK(hipEventRecord(ev[0], ctx().stream));
+  const T *omd = omegat;
+  Arg<T> O(use_seed ? nullptr : omegat, use_seed ? 0 : span(d.Nl, d.n, ldo), true, false);
+  if (use_seed) {
+    om.alloc(d.Nl * d.n);
+    gaussian_omega<T>(d.Nl, d.n, om.get(), d.Nl, seed);
+    omd = om.get();
+    ldo = d.Nl;
+  } else {
+    omd = O.dev;
+  }
+  stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
+  HIP_CHECK(hipEventRecord(ev[1], ctx().stream));
+  stage_msg("done.\nOrthogonalizing ............... ");
+  if (ao[0] == 'T') stage_orth_tsqr<T>(d, yst.get(), d.Nl);
+  else stage_orth_gramian<T>(d, yst.get(), d.Nl);
+  HIP_CHECK(hipEventRecord(ev[2], ctx().stream));
+  stage_msg("done.\nIntegrating ................... ");
+  if (kn) stage_integrate_kn<T>(d, ka, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
+  else stage_integrate_wy<T>(d, wa, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
+  HIP_CHECK(hipEventRecord(ev[3], ctx().stream));
+  stage_msg("done.\nPostprocessing ................ ");
+  if (ap[0] == 'T')
+    stage_post_tsqr<T>(d, ordera_, A.dev, lda, qt.get(), d.l, S.dev, U.dev, ldut, V.dev, ldvt,
+                       want_u, want_v);
+  else
+    stage_post_gramian<T>(d, ordera_, A.dev, lda, qt.get(), d.l, S.dev, U.dev, ldut, V.dev, ldvt,
+                          want_u, want_v);
+  HIP_CHECK(hipEventRecord(ev[4], ctx().stream));
+  S.finish();
+  U.finish();
+  V.finish();
+  sync();
+  stage_msg("done.\n");
+  if (time) {
+    for (int q = 0; q < 4; ++q) {
+      float ms = 0;
+      HIP_CHECK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+      time[q] = ms * 1e-3;
+    }
+  }
+  for (auto &e : ev) (void)hipEventDestroy(e);
+  if (retv && retc) {
+    if (retc[2] > 0 && retv[2]) retv[2][0] = (T)it;
+    if (retc[2] > 1 && retv[2]) retv[2][1] = (T)err;
+  }
+}
+
+}  // namespace isvd
+
+using namespace isvd;
+
+// =============================================================================================
+// extern "C"
+// =============================================================================================
+extern "C" {
+
+const char *isvd_GetLastError(void) { return g_last_error.c_str(); }
+int isvd_GetErrorCount(void) { return g_error_count; }
+
+isvd_Param isvd_createParam(const isvd_int_t nrow, const isvd_int_t ncol, const isvd_int_t rank,
+                            const isvd_int_t over_rank, const isvd_int_t num_sketch,
+                            const mpi_int_t mpi_root, const isvd_MpiComm mpi_comm) {
+  // src/lib/libisvd/core/param.c:37-80 with P = 1 (the m <= n assert of :43 is not applied)
+  if (!(nrow > 0 && ncol > 0 && rank > 0 && over_rank >= 0 && num_sketch > 0)) {
+    std::fprintf(stderr, "isvd: isvd_createParam: invalid sizes (m=%d n=%d k=%d p=%d N=%d)\n",
+                 (int)nrow, (int)ncol, (int)rank, (int)over_rank, (int)num_sketch);
+    g_last_error = "isvd_createParam: invalid sizes";
+    ++g_error_count;
+    const char *ab = std::getenv("ISVD_ASSERT_ABORT");
+    if (ab && ab[0] == '1') std::abort();
+  }
+  isvd_Param p = {mpi_comm, 1, mpi_root, 0, nrow, ncol, nrow, ncol, nrow, ncol, nrow, ncol, 0,
+                  nrow, 0, ncol, rank, over_rank, rank + over_rank,
+                  (rank + over_rank) * num_sketch, num_sketch};
+  return p;
+}
+
+void isvd_Init_cpu(int *, char ***, const isvd_MpiComm) {}
+void isvd_Finalize_cpu(void) {}
+void isvd_PrintEnvironment_cpu(const isvd_MpiComm) {
+  std::printf("iSVD (MI355X native) -- one process, index width %d bits\n",
+              (int)(8 * sizeof(isvd_int_t)));
+}
+
+void isvd_Init_gpu(int *, char ***, const isvd_MpiComm) {
+  guarded([] { ensure_init(); });
+}
+
+void isvd_Finalize_gpu(void) {
+  guarded([] {
+    if (g_init) {
+      HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+      HIP_CHECK(hipStreamDestroy(g_ctx.stream));
+      g_ctx.stream = nullptr;
+      g_init = false;
+    }
+  });
+}
+
+void isvd_PrintEnvironment_gpu(const isvd_MpiComm) {
+  guarded([] {
+    ensure_init();
+    hipDeviceProp_t prop;
+    HIP_CHECK(hipGetDeviceProperties(&prop, g_ctx.device));
+    std::printf("GPU %d: %s (%s), %d CUs, %.1f GiB HBM, LDS %zu KiB per workgroup\n",
+                g_ctx.device, prop.name, prop.gcnArchName, prop.multiProcessorCount,
+                prop.totalGlobalMem / 1073741824.0, prop.sharedMemPerBlock / 1024);
+  });
+}
+
+isvd_s_val_t *isvd_sMalloc(const size_t num) {
+  void *p = nullptr;
+  if (posix_memalign(&p, 64, num * sizeof(isvd_s_val_t) + 1) != 0) return nullptr;
+  return static_cast<isvd_s_val_t *>(p);
+}
+isvd_d_val_t *isvd_dMalloc(const size_t num) {
+  void *p = nullptr;
+  if (posix_memalign(&p, 64, num * sizeof(isvd_d_val_t) + 1) != 0) return nullptr;
+  return static_cast<isvd_d_val_t *>(p);
+}
+void isvd_Free(void *ptr) { std::free(ptr); }
+
+#define ISVD_DEFINE(x, T)                                                                      \
+  void isvd_##x##Isvd(const isvd_Param param, const char *alg_s, const char *alg_o,           \
+                      const char *alg_i, const char *alg_p, const T *argv[4],                 \
+                      const isvd_int_t argc[4], T *retv[4], const isvd_int_t retc[4],         \
+                      double time[4], FILE *stream, const char dista, const char ordera,      \
+                      const T *a, const isvd_int_t lda, T *s, T *ut, const isvd_int_t ldut,   \
+                      T *vt, const isvd_int_t ldvt, const isvd_int_t seed,                    \
+                      const mpi_int_t ut_root, const mpi_int_t vt_root) {                     \
+    guarded([&] {                                                                             \
+      isvd_impl<T>(param, alg_s, alg_o, alg_i, alg_p, argv, argc, retv, retc, time, stream,   \
+                   dista, ordera, a, lda, nullptr, 0, s, ut, ldut, vt, ldvt, true,            \
+                   (unsigned long long)(long long)seed, ut_root, vt_root);                    \
+    });                                                                                       \
+  }                                                                                           \
+  void isvd_##x##IsvdOmega(const isvd_Param param, const char *alg_s, const char *alg_o,      \
+                           const char *alg_i, const char *alg_p, const T *argv[4],            \
+                           const isvd_int_t argc[4], T *retv[4], const isvd_int_t retc[4],    \
+                           double time[4], FILE *stream, const char dista, const char ordera, \
+                           const T *a, const isvd_int_t lda, const T *omegat,                 \
+                           const isvd_int_t ldomegat, T *s, T *ut, const isvd_int_t ldut,     \
+                           T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,             \
+                           const mpi_int_t vt_root) {                                         \
+    guarded([&] {                                                                             \
+      isvd_impl<T>(param, alg_s, alg_o, alg_i, alg_p, argv, argc, retv, retc, time, stream,   \
+                   dista, ordera, a, lda, omegat, ldomegat, s, ut, ldut, vt, ldvt, false, 0,  \
+                   ut_root, vt_root);                                                         \
+    });                                                                                       \
+  }                                                                                           \
+  void isvd_##x##SketchGaussianProjection(                                                     \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, T *yst, const isvd_int_t ldyst, const isvd_int_t seed,            \
+      const mpi_int_t mpi_root) {                                                             \
+    (void)argv; (void)retv; (void)retc; (void)mpi_root;                                       \
+    if (argc < 0) return;                                                                     \
+    guarded([&] {                                                                             \
+      sketch_impl<T>(param, dista, ordera, a, lda, nullptr, 0, yst, ldyst, true,              \
+                     (unsigned long long)(long long)seed);                                    \
+    });                                                                                       \
+  }                                                                                           \
+  void isvd_##x##SketchGaussianProjection_gpu(                                                 \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, T *yst, const isvd_int_t ldyst, const isvd_int_t seed,            \
+      const mpi_int_t mpi_root) {                                                             \
+    isvd_##x##SketchGaussianProjection(param, argv, argc, retv, retc, dista, ordera, a, lda,  \
+                                       yst, ldyst, seed, mpi_root);                           \
+  }                                                                                           \
+  void isvd_##x##SketchGaussianProjectionOmega(                                                \
+      const isvd_Param param, const char dista, const char ordera, const T *a,                \
+      const isvd_int_t lda, const T *omegat, const isvd_int_t ldomegat, T *yst,               \
+      const isvd_int_t ldyst) {                                                               \
+    guarded([&] {                                                                             \
+      sketch_impl<T>(param, dista, ordera, a, lda, omegat, ldomegat, yst, ldyst, false, 0);   \
+    });                                                                                       \
+  }                                                                                           \
+  void isvd_##x##OrthogonalizeTallSkinnyQr(const isvd_Param param, const T *argv,             \
+                                           const isvd_int_t argc, T *retv,                    \
+                                           const isvd_int_t retc, T *yst,                     \
+                                           const isvd_int_t ldyst) {                          \
+    (void)argv; (void)retv; (void)retc;                                                       \
+    guarded([&] { orth_impl<T>(param, argc, yst, ldyst, true); });                            \
+  }                                                                                           \
+  void isvd_##x##OrthogonalizeGramian(const isvd_Param param, const T *argv,                  \
+                                      const isvd_int_t argc, T *retv, const isvd_int_t retc,  \
+                                      T *yst, const isvd_int_t ldyst) {                       \
+    (void)argv; (void)retv; (void)retc;                                                       \
+    guarded([&] { orth_impl<T>(param, argc, yst, ldyst, false); });                           \
+  }                                                                                           \
+  void isvd_##x##IntegrateKolmogorovNagumo(const isvd_Param param, const T *argv,             \
+                                           const isvd_int_t argc, T *retv,                    \
+                                           const isvd_int_t retc, const T *yst,               \
+                                           const isvd_int_t ldyst, T *qt,                     \
+                                           const isvd_int_t ldqt) {                           \
+    guarded([&] { kn_impl<T>(param, argv, argc, retv, retc, yst, ldyst, qt, ldqt); });        \
+  }                                                                                           \
+  void isvd_##x##IntegrateWenYin(const isvd_Param param, const T *argv, const isvd_int_t argc, \
+                                 T *retv, const isvd_int_t retc, const T *yst,                \
+                                 const isvd_int_t ldyst, T *qt, const isvd_int_t ldqt) {      \
+    guarded([&] { wy_impl<T>(param, argv, argc, retv, retc, yst, ldyst, qt, ldqt); });        \
+  }                                                                                           \
+  void isvd_##x##PostprocessTallSkinnyQr(                                                      \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root) {                                                              \
+    (void)argv; (void)retv; (void)retc;                                                       \
+    guarded([&] {                                                                             \
+      post_impl<T>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
+                   ut_root, vt_root, true);                                                   \
+    });                                                                                       \
+  }                                                                                           \
+  void isvd_##x##PostprocessGramian(                                                           \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root) {                                                              \
+    (void)argv; (void)retv; (void)retc;                                                       \
+    guarded([&] {                                                                             \
+      post_impl<T>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
+                   ut_root, vt_root, false);                                                  \
+    });                                                                                       \
+  }                                                                                           \
+  void isvd_##x##PostprocessTallSkinnyQr_gpu(                                                  \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root) {                                                              \
+    isvd_##x##PostprocessTallSkinnyQr(param, argv, argc, retv, retc, dista, ordera, a, lda,   \
+                                      qt, ldqt, s, ut, ldut, vt, ldvt, ut_root, vt_root);     \
+  }                                                                                           \
+  void isvd_##x##PostprocessGramian_gpu(                                                       \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root) {                                                              \
+    isvd_##x##PostprocessGramian(param, argv, argc, retv, retc, dista, ordera, a, lda, qt,    \
+                                 ldqt, s, ut, ldut, vt, ldvt, ut_root, vt_root);              \
+  }
+
+ISVD_DEFINE(s, isvd_s_val_t)
+ISVD_DEFINE(d, isvd_d_val_t)
+
+}  // extern "C"

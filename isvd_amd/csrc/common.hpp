@@ -1,0 +1,103 @@
+// Common definitions for the MI355X (gfx950) iSVD library: error state, HIP checks,
+// device-buffer RAII and the per-call execution context.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+namespace isvd {
+
+using i64 = long long;
+
+// ---------------------------------------------------------------------------------------------
+// Error handling.  The reference prints and returns for bad selector strings / chars
+// (src/lib/libisvd/util/arg.h:40-48, :70-78) and asserts for everything else
+// (src/lib/libisvd/def.h:51-68).  Here every failure prints to stderr, records a message
+// readable through isvd_GetLastError(), and unwinds the call with an exception that the
+// C-ABI layer catches; ISVD_ASSERT_ABORT=1 restores the reference's abort().
+// ---------------------------------------------------------------------------------------------
+struct Error {
+  std::string msg;
+};
+
+void set_last_error(const std::string &msg);
+[[noreturn]] void fail(const std::string &msg);
+
+#define ISVD_CHECK(cond, ...)                                                   \
+  do {                                                                          \
+    if (!(cond)) {                                                              \
+      char buf_[512];                                                           \
+      std::snprintf(buf_, sizeof(buf_), __VA_ARGS__);                           \
+      ::isvd::fail(std::string(__func__) + ": " + buf_);                        \
+    }                                                                           \
+  } while (0)
+
+#define HIP_CHECK(expr)                                                         \
+  do {                                                                          \
+    hipError_t e_ = (expr);                                                     \
+    if (e_ != hipSuccess) {                                                     \
+      char buf_[512];                                                           \
+      std::snprintf(buf_, sizeof(buf_), "%s failed: %s (%s) at %s:%d", #expr,   \
+                    hipGetErrorName(e_), hipGetErrorString(e_), __FILE__,       \
+                    __LINE__);                                                  \
+      ::isvd::fail(buf_);                                                       \
+    }                                                                           \
+  } while (0)
+
+// Launch check: catches invalid launch configurations right away.
+#define LAUNCH_CHECK() HIP_CHECK(hipGetLastError())
+
+// ---------------------------------------------------------------------------------------------
+// Execution context: one HIP stream per library call (default priority), stream-ordered
+// allocations from the device's default memory pool.
+// ---------------------------------------------------------------------------------------------
+struct Ctx {
+  hipStream_t stream = nullptr;
+  int device = 0;
+};
+
+Ctx &ctx();                // library context (created by isvd_Init_gpu or lazily)
+void ensure_init();         // selects device 0, creates the stream
+
+// Device buffer owned by the caller's scope, allocated stream-ordered.
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  i64 n = 0;
+  DBuf() = default;
+  explicit DBuf(i64 count) { alloc(count); }
+  void alloc(i64 count) {
+    release();
+    n = count;
+    if (count > 0) {
+      HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&p), sizeof(T) * (size_t)count,
+                               ctx().stream));
+    }
+  }
+  void release() {
+    if (p) {
+      (void)hipFreeAsync(p, ctx().stream);
+      p = nullptr;
+    }
+    n = 0;
+  }
+  ~DBuf() { release(); }
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  T *get() const { return p; }
+};
+
+// True when ptr is device (HBM) memory usable in place by kernels on the current device.
+bool is_device_pointer(const void *ptr);
+
+void sync();  // hipStreamSynchronize(ctx().stream) with error check
+
+inline int ceil_div(i64 a, i64 b) { return (int)((a + b - 1) / b); }
+
+}  // namespace isvd

@@ -1,0 +1,323 @@
+// MFMA GEMM for gfx950 (CDNA4), FP64 and FP32, column-major operands.
+//
+// One kernel template covers the four transpose combinations.  Tiles are staged through
+// LDS in k-major layout ([BK][BM] for op(A), [BK][BN] for op(B)), so the MFMA operand
+// fragments (lane -> (row = lane%16, k = lane/16)) are read with unit stride.  Global loads
+// walk the contiguous dimension of each operand, so both the "N" and the "T" forms load
+// coalesced.  A register-prefetch double buffer overlaps the next tile's global loads with
+// the current tile's MFMAs (one barrier per K tile).
+#include "la.hpp"
+
+namespace isvd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<double> {
+  using acc_t = d4;
+  // v_mfma_f64_16x16x4f64: A lane holds A[lane%16][lane/16], B lane holds B[lane/16][lane%16];
+  // D register g of a lane holds D[lane/16 + 4g][lane%16].
+  static __device__ __forceinline__ acc_t run(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int g) { return (lane >> 4) + 4 * g; }
+};
+
+template <>
+struct Mfma<float> {
+  using acc_t = f4;
+  // v_mfma_f32_16x16x4f32: same operand layout; D register g holds D[4*(lane/16) + g][lane%16].
+  static __device__ __forceinline__ acc_t run(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int lane, int g) { return 4 * (lane >> 4) + g; }
+};
+
+template <typename T>
+struct GemmArgs {
+  i64 m, n, k;
+  T alpha, beta;
+  const T *A;
+  i64 lda;
+  const T *B;
+  i64 ldb;
+  T *C;
+  i64 ldc;
+  i64 sA, sB, sC;
+  int ksplit;    // number of K chunks (grid.y = batch * ksplit)
+  i64 kchunk;    // K elements per chunk (multiple of BK)
+  T *W;          // split-K workspace: [batch][ksplit][m*n] dense col-major partials
+  int group_m;   // raster group height (tiles) for L2 reuse
+};
+
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN>
+__global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs<T> g) {
+  constexpr int NT = 64 * WGM * WGN;
+  constexpr int PAD = (sizeof(T) == 8) ? 2 : 4;
+  constexpr int LDA_S = BM + PAD;
+  constexpr int LDB_S = BN + PAD;
+  constexpr int A_LD = BM * BK / NT;  // elements per thread per tile
+  constexpr int B_LD = BK * BN / NT;
+  static_assert(A_LD * NT == BM * BK && B_LD * NT == BK * BN, "tile/threads mismatch");
+  constexpr int WTM = BM / WGM, WTN = BN / WGN;
+  constexpr int FM = WTM / 16, FN = WTN / 16;
+  using acc_t = typename Mfma<T>::acc_t;
+
+  __shared__ T As[2][BK * LDA_S];
+  __shared__ T Bs[2][BK * LDB_S];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave % WGM, wn = wave / WGM;
+
+  // ---- tile coordinates (grouped raster) ----
+  const i64 tiles_m = (g.m + BM - 1) / BM;
+  const i64 tiles_n = (g.n + BN - 1) / BN;
+  const i64 pid = blockIdx.x;
+  const i64 in_group = (i64)g.group_m * tiles_n;
+  const i64 group_id = pid / in_group;
+  const i64 first_m = group_id * g.group_m;
+  const i64 gsz = (tiles_m - first_m) < g.group_m ? (tiles_m - first_m) : g.group_m;
+  const i64 tm = first_m + (pid % in_group) % gsz;
+  const i64 tn = (pid % in_group) / gsz;
+  const i64 m0 = tm * BM, n0 = tn * BN;
+
+  const int batch_id = blockIdx.y / g.ksplit;
+  const int ks = blockIdx.y % g.ksplit;
+  const i64 kbeg = (i64)ks * g.kchunk;
+  i64 kend = kbeg + g.kchunk;
+  if (kend > g.k) kend = g.k;
+
+  const T *__restrict__ Ab = g.A + (i64)batch_id * g.sA;
+  const T *__restrict__ Bb = g.B + (i64)batch_id * g.sB;
+
+  T ra[A_LD], rb[B_LD];
+
+  auto load_tiles = [&](i64 k0) {
+#pragma unroll
+    for (int r = 0; r < A_LD; ++r) {
+      const int e = tid + NT * r;
+      int i, p;
+      if (!TA) { i = e % BM; p = e / BM; } else { i = e / BK; p = e % BK; }
+      const i64 gi = m0 + i, gp = k0 + p;
+      T v = T(0);
+      if (gi < g.m && gp < kend) v = TA ? Ab[gp + gi * g.lda] : Ab[gi + gp * g.lda];
+      ra[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < B_LD; ++r) {
+      const int e = tid + NT * r;
+      int j, p;
+      if (TB) { j = e % BN; p = e / BN; } else { j = e / BK; p = e % BK; }
+      const i64 gj = n0 + j, gp = k0 + p;
+      T v = T(0);
+      if (gj < g.n && gp < kend) v = TB ? Bb[gj + gp * g.ldb] : Bb[gp + gj * g.ldb];
+      rb[r] = v;
+    }
+  };
+  auto store_tiles = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < A_LD; ++r) {
+      const int e = tid + NT * r;
+      int i, p;
+      if (!TA) { i = e % BM; p = e / BM; } else { i = e / BK; p = e % BK; }
+      As[buf][p * LDA_S + i] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < B_LD; ++r) {
+      const int e = tid + NT * r;
+      int j, p;
+      if (TB) { j = e % BN; p = e / BN; } else { j = e / BK; p = e % BK; }
+      Bs[buf][p * LDB_S + j] = rb[r];
+    }
+  };
+
+  acc_t acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+
+  const i64 nk = (kend > kbeg) ? (kend - kbeg + BK - 1) / BK : 0;
+  if (nk > 0) {
+    load_tiles(kbeg);
+    store_tiles(0);
+    __syncthreads();
+  }
+  const int lr = lane & 15, lk = lane >> 4;
+  for (i64 kt = 0; kt < nk; ++kt) {
+    const int buf = (int)(kt & 1);
+    if (kt + 1 < nk) load_tiles(kbeg + (kt + 1) * BK);
+    const T *as = As[buf];
+    const T *bs = Bs[buf];
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      T fa[FM], fb[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) fa[a] = as[(kk + lk) * LDA_S + wm * WTM + a * 16 + lr];
+#pragma unroll
+      for (int b = 0; b < FN; ++b) fb[b] = bs[(kk + lk) * LDB_S + wn * WTN + b * 16 + lr];
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = Mfma<T>::run(fa[a], fb[b], acc[a][b]);
+    }
+    if (kt + 1 < nk) store_tiles(buf ^ 1);
+    __syncthreads();
+  }
+
+  // ---- epilogue ----
+  if (g.ksplit == 1) {
+    T *Cb = g.C + (i64)batch_id * g.sC;
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const i64 r = m0 + wm * WTM + a * 16 + Mfma<T>::row(lane, q);
+          const i64 c = n0 + wn * WTN + b * 16 + lr;
+          if (r < g.m && c < g.n) {
+            T v = g.alpha * acc[a][b][q];
+            if (g.beta != T(0)) v += g.beta * Cb[r + c * g.ldc];
+            Cb[r + c * g.ldc] = v;
+          }
+        }
+  } else {
+    T *Wb = g.W + ((i64)batch_id * g.ksplit + ks) * g.m * g.n;
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const i64 r = m0 + wm * WTM + a * 16 + Mfma<T>::row(lane, q);
+          const i64 c = n0 + wn * WTN + b * 16 + lr;
+          if (r < g.m && c < g.n) Wb[r + c * g.m] = acc[a][b][q];
+        }
+  }
+}
+
+// C = alpha * sum_s W[s] + beta * C  (fixed summation order -> deterministic).
+template <typename T>
+__global__ void splitk_reduce_kernel(i64 m, i64 n, int ksplit, const T *__restrict__ W,
+                                     T alpha, T beta, T *__restrict__ C, i64 ldc, i64 sC) {
+  const int b = blockIdx.y;
+  const i64 mn = m * n;
+  const T *Wb = W + (i64)b * ksplit * mn;
+  T *Cb = C + (i64)b * sC;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < mn;
+       e += (i64)gridDim.x * blockDim.x) {
+    T s = T(0);
+    for (int q = 0; q < ksplit; ++q) s += Wb[q * mn + e];
+    const i64 r = e % m, c = e / m;
+    T v = alpha * s;
+    if (beta != T(0)) v += beta * Cb[r + c * ldc];
+    Cb[r + c * ldc] = v;
+  }
+}
+
+template <typename T>
+__global__ void scale_kernel(i64 m, i64 n, T beta, T *C, i64 ldc, i64 sC) {
+  T *Cb = C + (i64)blockIdx.y * sC;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < m * n;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 r = e % m, c = e / m;
+    Cb[r + c * ldc] = (beta == T(0)) ? T(0) : beta * Cb[r + c * ldc];
+  }
+}
+
+namespace {
+
+constexpr int kTargetBlocks = 1024;  // ~4 workgroups per CU on 256 CUs
+
+template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN>
+void launch_cfg(GemmArgs<T> g, i64 batch) {
+  const i64 tiles = ((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN);
+  // split-K when the output alone cannot fill the device
+  int ksplit = 1;
+  if (tiles * batch < kTargetBlocks && g.k >= 8 * BK) {
+    i64 want = (kTargetBlocks + tiles * batch - 1) / (tiles * batch);
+    i64 maxs = g.k / (4 * BK);
+    if (want > maxs) want = maxs;
+    if (want > 64) want = 64;
+    if (want < 1) want = 1;
+    ksplit = (int)want;
+  }
+  i64 kchunk = (g.k + ksplit - 1) / ksplit;
+  kchunk = (kchunk + BK - 1) / BK * BK;
+  ksplit = (int)((g.k + kchunk - 1) / kchunk);
+  if (ksplit < 1) ksplit = 1;
+  g.ksplit = ksplit;
+  g.kchunk = (ksplit == 1) ? (g.k > 0 ? g.k : 1) : kchunk;
+  g.group_m = 8;
+  DBuf<T> work;
+  if (ksplit > 1) {
+    work.alloc(batch * ksplit * g.m * g.n);
+    g.W = work.get();
+  }
+  ISVD_CHECK(batch * ksplit <= 65535, "grid.y too large (batch %lld x ksplit %d)", batch, ksplit);
+  dim3 grid((unsigned)tiles, (unsigned)(batch * ksplit));
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, BM, BN, BK, WGM, WGN>), grid, dim3(64 * WGM * WGN),
+                     0, ctx().stream, g);
+  LAUNCH_CHECK();
+  if (ksplit > 1) {
+    const i64 mn = g.m * g.n;
+    int blocks = (int)((mn + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3(blocks, (unsigned)batch), dim3(256), 0,
+                       ctx().stream, g.m, g.n, ksplit, (const T *)g.W, g.alpha, g.beta, g.C,
+                       g.ldc, g.sC);
+    LAUNCH_CHECK();
+  }
+}
+
+template <typename T, bool TA, bool TB>
+void launch(GemmArgs<T> g, i64 batch) {
+  const bool big = (g.m >= 128 && g.n >= 128);
+  if (big)
+    launch_cfg<T, TA, TB, 128, 128, 16, 2, 2>(g, batch);
+  else
+    launch_cfg<T, TA, TB, 64, 64, 16, 2, 2>(g, batch);
+}
+
+}  // namespace
+
+template <typename T>
+void gemm(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda,
+          const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch, i64 sA, i64 sB, i64 sC) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  const bool ta = (transa == 'T' || transa == 't' || transa == 'C' || transa == 'c');
+  const bool tb = (transb == 'T' || transb == 't' || transb == 'C' || transb == 'c');
+  ISVD_CHECK(ldc >= m, "ldc (%lld) < m (%lld)", ldc, m);
+  if (k <= 0 || alpha == T(0)) {
+    int blocks = (int)((m * n + 255) / 256);
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(scale_kernel<T>, dim3(blocks, (unsigned)batch), dim3(256), 0,
+                       ctx().stream, m, n, beta, C, ldc, sC);
+    LAUNCH_CHECK();
+    return;
+  }
+  ISVD_CHECK(lda >= (ta ? k : m), "lda (%lld) too small", lda);
+  ISVD_CHECK(ldb >= (tb ? n : k), "ldb (%lld) too small", ldb);
+  GemmArgs<T> g{};
+  g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.sA = sA; g.sB = sB; g.sC = sC; g.W = nullptr;
+  if (!ta && !tb) launch<T, false, false>(g, batch);
+  else if (!ta && tb) launch<T, false, true>(g, batch);
+  else if (ta && !tb) launch<T, true, false>(g, batch);
+  else launch<T, true, true>(g, batch);
+}
+
+template void gemm<double>(char, char, i64, i64, i64, double, const double *, i64,
+                           const double *, i64, double, double *, i64, i64, i64, i64, i64);
+template void gemm<float>(char, char, i64, i64, i64, float, const float *, i64, const float *,
+                          i64, float, float *, i64, i64, i64, i64, i64);
+
+}  // namespace isvd

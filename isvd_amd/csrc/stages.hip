@@ -1,0 +1,489 @@
+// The four iSVD stages on one GPU.  Each function cites the reference code it follows.
+#include "stages.hpp"
+
+#include <cmath>
+#include <cstdlib>
+
+#include "la.hpp"
+
+namespace isvd {
+
+namespace {
+
+// ---------------------------------------------------------------------------------------------
+// Philox-4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) counter-based generator + Box-Muller.
+// Element e of Omega^T (column-major Nl x n) uses counter (e, 0, 0, 0) and key (seed_lo,
+// seed_hi); the first two 32-bit words make u1, the last two u2 (53-bit uniforms in (0, 1]).
+// ---------------------------------------------------------------------------------------------
+__device__ __forceinline__ void philox_round(unsigned (&c)[4], unsigned (&k)[2]) {
+  const unsigned long long p0 = 0xD2511F53ull * c[0];
+  const unsigned long long p1 = 0xCD9E8D57ull * c[2];
+  const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+  const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+  const unsigned n0 = hi1 ^ c[1] ^ k[0];
+  const unsigned n2 = hi0 ^ c[3] ^ k[1];
+  c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  k[0] += 0x9E3779B9u;
+  k[1] += 0xBB67AE85u;
+}
+
+template <typename T>
+__global__ void gaussian_kernel(i64 rows, i64 cols, T *out, i64 ld, unsigned long long seed) {
+  const i64 total = rows * cols;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < total;
+       e += (i64)gridDim.x * blockDim.x) {
+    unsigned c[4] = {(unsigned)e, (unsigned)(e >> 32), 0u, 0u};
+    unsigned k[2] = {(unsigned)seed, (unsigned)(seed >> 32)};
+#pragma unroll
+    for (int r = 0; r < 10; ++r) philox_round(c, k);
+    const unsigned long long a = ((unsigned long long)c[0] << 32) | c[1];
+    const unsigned long long b = ((unsigned long long)c[2] << 32) | c[3];
+    const double u1 = ((a >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double u2 = ((b >> 11) + 1) * (1.0 / 9007199254740992.0);
+    const double z = sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925 * u2);
+    const i64 i = e % rows, j = e / rows;
+    out[i + j * ld] = (T)z;
+  }
+}
+
+// W[:, j] *= 1 / sqrt(s[j])  for j < ncols (batched; vSqrt + Dism of the Gramian stage)
+template <typename T>
+__global__ void sqrt_and_colscale_kernel(int l, int ncols, const T *s, i64 lds, T *W, i64 ldw,
+                                         i64 sW) {
+  const int b = blockIdx.y;
+  const T *sb = s + (i64)b * lds;
+  T *Wb = W + (i64)b * sW;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < l * ncols;
+       e += gridDim.x * blockDim.x) {
+    const int i = e % l, j = e / l;
+    Wb[i + (i64)j * ldw] *= T(1) / sqrt(sb[j]);
+  }
+}
+
+template <typename T>
+__global__ void sqrt_inplace_kernel(int n, T *s) {
+  const int b = blockIdx.y;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x)
+    s[(i64)b * n + e] = sqrt(s[(i64)b * n + e]);
+}
+
+// KN (@x@_integrate_kolmogorov_nagumo.c:196-212, :230-236): from eig(Z) = (w, V):
+//   s = sqrt(1/2 + sqrt(1/4 - w)); ss = sqrt(s); ZS = V diag(ss); ZI = V diag(1/ss);
+//   err = ||s - 1||_2 (one workgroup, fixed reduction order)
+template <typename T>
+__global__ __launch_bounds__(256) void kn_scalars_kernel(int l, const T *w, const T *V, T *ZS,
+                                                         T *ZI, T *err) {
+  __shared__ T ss[128];
+  __shared__ T red[4];
+  T part = 0;
+  for (int j = threadIdx.x; j < l; j += 256) {
+    const T s = sqrt(T(0.5) + sqrt(T(0.25) - w[j]));
+    ss[j] = sqrt(s);
+    part += (s - T(1)) * (s - T(1));
+  }
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) err[0] = sqrt(red[0] + red[1] + red[2] + red[3]);
+  for (int e = threadIdx.x; e < l * l; e += 256) {
+    const int j = e / l;
+    ZS[e] = V[e] * ss[j];
+    ZI[e] = V[e] / ss[j];
+  }
+}
+
+// A := upper(A) mirrored to the lower triangle (syrk 'U' result used through symm 'U')
+template <typename T>
+__global__ void sym_from_upper_kernel(int n, T *A, i64 lda) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
+    const int i = e % n, j = e / n;
+    if (i > j) A[i + (i64)j * lda] = A[j + (i64)i * lda];
+  }
+}
+
+// WY step operators in closed form.  With C = [Dc/2 - I/tau, I/2; -Dgc/2, -Dc/2 - I/tau]
+// (@x@_integrate_wen_yin.c:249-259), eliminating with the C_12 = I/2 block gives
+//   Fgc = [inv(C)_12 Dc - inv(C)_11] = -(2/tau) S^-1,
+//   Fc  = [inv(C)_22 Dc - inv(C)_21] + I = (2/tau) (Dc - (2/tau) I) S^-1 - I,
+// with S = -(Dgc - Dc^2)/2 - (2/tau^2) I (C is singular exactly when S is).  S^-1 comes from
+// one eigendecomposition Dgc - Dc^2 = V diag(w) V^T per iteration: S^-1 = V diag(mu) V^T,
+// mu_i = 1 / (-w_i/2 - 2/tau^2), so each line-search trip costs l x l products only.
+template <typename T>
+__global__ void wy_vscale_kernel(int l, const T *w, const T *V, T inv_tau, T *VS) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < l * l; e += gridDim.x * blockDim.x) {
+    const int j = e / l;
+    const T mu = T(1) / (T(-0.5) * w[j] - T(2) * inv_tau * inv_tau);
+    VS[e] = V[e] * mu;
+  }
+}
+
+// F := F + a * S + b * I   (l x l)
+template <typename T>
+__global__ void axpy_diag_kernel(int l, T *F, const T *S, T a, T b) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < l * l; e += gridDim.x * blockDim.x) {
+    const int i = e % l, j = e / l;
+    F[e] += a * S[e] + ((i == j) ? b : T(0));
+  }
+}
+
+// sum |diag(A)| - ||B||_F^2  (mu of @x@_integrate_wen_yin.c:232, :307); one workgroup
+template <typename T>
+__global__ __launch_bounds__(256) void wy_mu_kernel(int l, const T *dgc, const T *dc, T *mu) {
+  __shared__ T red[4];
+  T part = 0;
+  for (int e = threadIdx.x; e < l * l; e += 256) {
+    const int i = e % l, j = e / l;
+    if (i == j) part += fabs(dgc[e]);
+    part -= dc[e] * dc[e];
+  }
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) mu[0] = red[0] + red[1] + red[2] + red[3];
+}
+
+// c = a - b (elementwise, n entries)
+template <typename T>
+__global__ void sub_kernel(i64 n, const T *a, const T *b, T *c) {
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < n;
+       e += (i64)gridDim.x * blockDim.x)
+    c[e] = a[e] - b[e];
+}
+
+template <typename T>
+__global__ void diag_add_kernel(int n, T *A, i64 lda, T v) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) A[i + (i64)i * lda] += v;
+}
+
+// Dism side 'R' (src/lib/libisvd/la/blas/dism.c:26-31): W[:, j] *= 1 / s[j]
+template <typename T>
+__global__ void colscale_inv_kernel(int rows, int ncols, const T *s, T *W, i64 ldw) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < rows * ncols;
+       e += gridDim.x * blockDim.x) {
+    const int i = e % rows, j = e / rows;
+    W[i + (i64)j * ldw] *= T(1) / s[j];
+  }
+}
+
+template <typename T>
+__global__ void transpose_small_kernel(int n, const T *A, i64 lda, T *B, i64 ldb) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
+    const int i = e % n, j = e / n;
+    B[j + (i64)i * ldb] = A[i + (i64)j * lda];
+  }
+}
+
+template <typename T>
+T read_scalar(const T *dptr) {
+  T h;
+  HIP_CHECK(hipMemcpyAsync(&h, dptr, sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
+  sync();
+  return h;
+}
+
+int grid_for(i64 n) {
+  i64 b = (n + 255) / 256;
+  if (b > 8192) b = 8192;
+  if (b < 1) b = 1;
+  return (int)b;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------------------
+template <typename T>
+void gaussian_omega(i64 rows, i64 cols, T *omegat, i64 ld, unsigned long long seed) {
+  hipLaunchKernelGGL(gaussian_kernel<T>, dim3(grid_for(rows * cols)), dim3(256), 0,
+                     ctx().stream, rows, cols, omegat, ld, seed);
+  LAUNCH_CHECK();
+}
+
+// Sketch: Y^T := Omega^T * op(A)^T   (sketchBlockRow, src/lib/libisvd/core/stage/@x@_sketch.h:
+// 136-139; sketchBlockCol :74-82 is the same product at P = 1).
+template <typename T>
+void stage_sketch(const Dims &d, char ordera, const T *a, i64 lda, const T *omegat, i64 ldo,
+                  T *yst, i64 ldyst) {
+  const char transa = (ordera == 'C') ? 'T' : 'N';
+  gemm<T>('N', transa, d.Nl, d.m, d.n, T(1), omegat, ldo, a, lda, T(0), yst, ldyst);
+}
+
+// Gramian orthogonalization (src/lib/libisvd/core/stage/@x@_orthogonalize_gramian.c:81-98).
+template <typename T>
+void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst) {
+  const i64 l = d.l, N = d.N, m = d.m;
+  DBuf<T> w(l * d.Nl), u(l * d.Nl), s(l * N), ycopy(ldyst * m);
+  // W_i := Y_i^T Y_i  (:81-84)
+  gemm<T>('N', 'T', l, l, m, T(1), yst, ldyst, yst, ldyst, T(0), w.get(), l, N, l, l, l * l);
+  // gesvd('O','N') of every W_i  (:87-90)
+  svd_small<T>(l, w.get(), l, l * l, s.get(), l, u.get(), l, l * l, nullptr, 0, 0, N);
+  // s := sqrt(s); U_i := U_i / s_i  (:91, :94)
+  hipLaunchKernelGGL(sqrt_and_colscale_kernel<T>, dim3(grid_for(l * l), (unsigned)N), dim3(256),
+                     0, ctx().stream, (int)l, (int)l, (const T *)s.get(), l, u.get(), l, l * l);
+  LAUNCH_CHECK();
+  // Q_i^T := U_i^T Y_i^T  (:95-98)
+  HIP_CHECK(hipMemcpyAsync(ycopy.get(), yst, sizeof(T) * ldyst * m, hipMemcpyDeviceToDevice,
+                           ctx().stream));
+  gemm<T>('T', 'N', l, m, l, T(1), u.get(), l, ycopy.get(), ldyst, T(0), yst, ldyst, N, l * l, l,
+          l);
+}
+
+// Tall-skinny QR orthogonalization (@x@_orthogonalize_tall_skinny_qr.c:127-200): every sketch
+// Y_i (row-major m x l inside yst) is replaced by the orthonormal factor of its QR.
+template <typename T>
+void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst) {
+  tsqr_rowmajor<T>(d.m, d.l, yst, ldyst, d.l, d.N, nullptr, 0, 0);
+}
+
+// Kolmogorov-Nagumo integration (@x@_integrate_kolmogorov_nagumo.c:155-276), iterated in the
+// coordinates of the sketch basis: every iterate satisfies Q_c = Qs X_c, and
+//   Bg = Qs^T (Qs B / N) = M B / N with M = Qs^T Qs (Nl x Nl, formed once),
+//   X_+ = X_c F + (B_c / N) C^-1   (the image of Q_+ = Q_c F + G_c C^-1),
+// so each iteration touches only Nl x l and l x l matrices; Qbar = Qs X is formed at the end.
+template <typename T>
+void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldyst, T *qt,
+                        i64 ldqt, i64 *iter_out, double *err_out) {
+  const i64 l = d.l, Nl = d.Nl, m = d.m;
+  const T ninv = T(1) / T(d.N);
+  DBuf<T> M(Nl * Nl), xc(Nl * l), xp(Nl * l), bc(Nl * l), bp(Nl * l), bgc(Nl * l);
+  DBuf<T> dc(l * l), z(l * l), w(l), zv(l * l), zs(l * l), zi(l * l), c(l * l), cinv(l * l),
+      err(1);
+  // M := Qs^T Qs
+  gemm<T>('N', 'T', Nl, Nl, m, T(1), yst, ldyst, yst, ldyst, T(0), M.get(), Nl);
+  // X_c := [I; 0] (Qc := Q_0, :156);  B_c := Qs^T Q_0 = M[:, 0:l]  (:159-160)
+  fill<T>(Nl, l, T(0), xc.get(), Nl, T(1));
+  omatcopy<T>('N', Nl, l, T(1), M.get(), Nl, bc.get(), Nl);
+  i64 it;
+  double error = -INFINITY;
+  for (it = 1;; ++it) {
+    // Bgc := M Bc / N  (:174-178)
+    gemm<T>('N', 'N', Nl, l, Nl, ninv, M.get(), Nl, bc.get(), Nl, T(0), bgc.get(), Nl);
+    // Dc := Bc^T Bc / N (:181); Z := Bc^T Bgc / N - Dc^T Dc (:184-190)
+    gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bc.get(), Nl, T(0), dc.get(), l);
+    gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bgc.get(), Nl, T(0), z.get(), l);
+    gemm<T>('T', 'N', l, l, l, T(-1), dc.get(), l, dc.get(), l, T(1), z.get(), l);
+    // eig(Z) (:193), scalars (:196-206)
+    syev_small<T>(l, z.get(), l, w.get(), zv.get(), l);
+    hipLaunchKernelGGL(kn_scalars_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
+                       (const T *)w.get(), (const T *)zv.get(), zs.get(), zi.get(), err.get());
+    LAUNCH_CHECK();
+    // C := ZS ZS^T (:209); inv(C) := ZI ZI^T, upper (:212)
+    gemm<T>('N', 'T', l, l, l, T(1), zs.get(), l, zs.get(), l, T(0), c.get(), l);
+    gemm<T>('N', 'T', l, l, l, T(1), zi.get(), l, zi.get(), l, T(0), cinv.get(), l);
+    hipLaunchKernelGGL(sym_from_upper_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
+                       ctx().stream, (int)l, cinv.get(), l);
+    LAUNCH_CHECK();
+    // Fc [in C] := C - Dc inv(C)  (:218)
+    gemm<T>('N', 'N', l, l, l, T(-1), dc.get(), l, cinv.get(), l, T(1), c.get(), l);
+    // X_+ := X_c Fc + (Bc / N) inv(C)  (image of :221-222)
+    gemm<T>('N', 'N', Nl, l, l, T(1), xc.get(), Nl, c.get(), l, T(0), xp.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, l, ninv, bc.get(), Nl, cinv.get(), l, T(1), xp.get(), Nl);
+    // B_+ := Bc Fc + Bgc inv(C)  (:225-226)
+    gemm<T>('N', 'N', Nl, l, l, T(1), bc.get(), Nl, c.get(), l, T(0), bp.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, cinv.get(), l, T(1), bp.get(), Nl);
+    // convergence (:230-241)
+    error = (double)read_scalar(err.get());
+    if (error <= args.tol) break;
+    if (it == args.maxit) {
+      ++it;
+      break;
+    }
+    std::swap(xc.p, xp.p);
+    std::swap(bc.p, bp.p);
+  }
+  // Qbar^T := X_+^T Qs^T  (:250-253)
+  gemm<T>('T', 'N', l, m, Nl, T(1), xp.get(), Nl, yst, ldyst, T(0), qt, ldqt);
+  *iter_out = it;
+  *err_out = error;
+}
+
+// Wen-Yin integration (@x@_integrate_wen_yin.c:201-384), same coordinates as KN:
+// Q_c = Qs xq, G_c = Qs gq (gq = B_c / N), X_c = Qs xx, <Qs a, Qs b> = <a, M b>; the
+// 2l x 2l inverse of the line search is replaced by its closed form (wy_vscale_kernel).
+template <typename T>
+void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldyst, T *qt,
+                        i64 ldqt, i64 *iter_out, double *mu_out) {
+  const i64 l = d.l, Nl = d.Nl, m = d.m;
+  const T ninv = T(1) / T(d.N);
+  const T half_ninv = T(0.5) / T(d.N);
+  DBuf<T> M(Nl * Nl), xq(Nl * l), xqp(Nl * l), xx(Nl * l), xxp(Nl * l), bc(Nl * l), bp(Nl * l),
+      bgc(Nl * l), gq(Nl * l), d1(Nl * l), d2(Nl * l), md1(Nl * l), md2(Nl * l);
+  DBuf<T> dc(l * l), dgc(l * l), zm(l * l), w(l), zv(l * l), vs(l * l), si(l * l), fcb(l * l),
+      fgcb(l * l), scal(4);
+  gemm<T>('N', 'T', Nl, Nl, m, T(1), yst, ldyst, yst, ldyst, T(0), M.get(), Nl);
+  // init (:201-235)
+  fill<T>(Nl, l, T(0), xq.get(), Nl, T(1));
+  omatcopy<T>('N', Nl, l, T(1), M.get(), Nl, bc.get(), Nl);
+  gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bc.get(), Nl, T(0), dc.get(), l);
+  omatcopy<T>('N', Nl, l, ninv, bc.get(), Nl, gq.get(), Nl);
+  gemm<T>('N', 'N', Nl, l, Nl, T(1), M.get(), Nl, gq.get(), Nl, T(0), bgc.get(), Nl);
+  gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bgc.get(), Nl, T(0), dgc.get(), l);
+  omatcopy<T>('N', Nl, l, T(1), gq.get(), Nl, xx.get(), Nl);
+  gemm<T>('N', 'N', Nl, l, l, T(-1), xq.get(), Nl, dc.get(), l, T(1), xx.get(), Nl);
+  hipLaunchKernelGGL(wy_mu_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
+                     (const T *)dgc.get(), (const T *)dc.get(), scal.get());
+  LAUNCH_CHECK();
+  double mu = (double)read_scalar(scal.get());
+  double taug = args.tau0, zeta = 1.0;
+  dot<T>(Nl * l, bc.get(), bc.get(), scal.get() + 1);
+  double phi = (double)half_ninv * (double)read_scalar(scal.get() + 1);
+  T *fc = fcb.get();
+  T *fgc = fgcb.get();
+  i64 it;
+  for (it = 1;; ++it) {
+    // step size (:243-282); eig(Dgc - Dc^2) once per iteration, then closed-form trips
+    gemm<T>('T', 'N', l, l, l, T(-1), dc.get(), l, dc.get(), l, T(0), zm.get(), l);
+    hipLaunchKernelGGL(axpy_diag_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
+                       (int)l, zm.get(), (const T *)dgc.get(), T(1), T(0));
+    LAUNCH_CHECK();
+    syev_small<T>(l, zm.get(), l, w.get(), zv.get(), l);
+    double tau = taug, phit = phi;
+    for (i64 tauiter = 1; tauiter <= args.taumaxit; ++tauiter, tau *= args.beta) {
+      const T it_ = (T)(1.0 / tau);
+      hipLaunchKernelGGL(wy_vscale_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
+                         (int)l, (const T *)w.get(), (const T *)zv.get(), it_, vs.get());
+      LAUNCH_CHECK();
+      // S^-1 = V diag(mu) V^T ; Fgc = -(2/tau) S^-1 ; Fc = (2/tau) Dc S^-1 - (4/tau^2) S^-1 - I
+      gemm<T>('N', 'T', l, l, l, T(1), vs.get(), l, zv.get(), l, T(0), si.get(), l);
+      gemm<T>('N', 'T', l, l, l, T(-2) * it_, vs.get(), l, zv.get(), l, T(0), fgc, l);
+      gemm<T>('N', 'N', l, l, l, T(2) * it_, dc.get(), l, si.get(), l, T(0), fc, l);
+      hipLaunchKernelGGL(axpy_diag_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
+                         (int)l, fc, (const T *)si.get(), T(-4) * it_ * it_, T(-1));
+      LAUNCH_CHECK();
+      // B+ := Bc Fc + Bgc Fgc  (:270-271)
+      gemm<T>('N', 'N', Nl, l, l, T(1), bc.get(), Nl, fc, l, T(0), bp.get(), Nl);
+      gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, fgc, l, T(1), bp.get(), Nl);
+      dot<T>(Nl * l, bp.get(), bp.get(), scal.get() + 2);
+      phit = (double)half_ninv * (double)read_scalar(scal.get() + 2);
+      if (phit >= phi + tau * args.sigma * mu) break;
+    }
+    // (:287-288)
+    phi = (args.eta * zeta * phi + phit) / (args.eta * zeta + 1);
+    zeta = args.eta * zeta + 1;
+    // Q+ := Qc Fc + Gc Fgc  (:291-292)
+    gemm<T>('N', 'N', Nl, l, l, T(1), xq.get(), Nl, fc, l, T(0), xqp.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, l, T(1), gq.get(), Nl, fgc, l, T(1), xqp.get(), Nl);
+    // D+ := B+^T B+ / N (:295); G+ := Qs B+ / N (:298); Bg+ := Qs^T G+ (:301-303)
+    gemm<T>('T', 'N', l, l, Nl, ninv, bp.get(), Nl, bp.get(), Nl, T(0), dc.get(), l);
+    omatcopy<T>('N', Nl, l, ninv, bp.get(), Nl, gq.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, Nl, T(1), M.get(), Nl, gq.get(), Nl, T(0), bgc.get(), Nl);
+    // Dg+ := B+^T Bg+ / N (:306); mu (:307)
+    gemm<T>('T', 'N', l, l, Nl, ninv, bp.get(), Nl, bgc.get(), Nl, T(0), dgc.get(), l);
+    hipLaunchKernelGGL(wy_mu_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
+                       (const T *)dgc.get(), (const T *)dc.get(), scal.get());
+    LAUNCH_CHECK();
+    mu = (double)read_scalar(scal.get());
+    // convergence (:312-319)
+    if (mu < args.tol * args.tol) break;
+    if (it == args.maxit) {
+      ++it;
+      break;
+    }
+    // X+ := G+ - Q+ D+ (:324-325); Delta1, Delta2 (:328-329)
+    omatcopy<T>('N', Nl, l, T(1), gq.get(), Nl, xxp.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, l, T(-1), xqp.get(), Nl, dc.get(), l, T(1), xxp.get(), Nl);
+    hipLaunchKernelGGL(sub_kernel<T>, dim3(grid_for(Nl * l)), dim3(256), 0, ctx().stream,
+                       Nl * l, (const T *)xq.get(), (const T *)xqp.get(), d1.get());
+    hipLaunchKernelGGL(sub_kernel<T>, dim3(grid_for(Nl * l)), dim3(256), 0, ctx().stream,
+                       Nl * l, (const T *)xx.get(), (const T *)xxp.get(), d2.get());
+    LAUNCH_CHECK();
+    // t (:334-342) in sketch coordinates: <D_a, D_b> = <d_a, M d_b>
+    gemm<T>('N', 'N', Nl, l, Nl, T(1), M.get(), Nl, d1.get(), Nl, T(0), md1.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, Nl, T(1), M.get(), Nl, d2.get(), Nl, T(0), md2.get(), Nl);
+    if (it % 2) {
+      dot<T>(Nl * l, d1.get(), md1.get(), scal.get());
+      dot<T>(Nl * l, d1.get(), md2.get(), scal.get() + 1);
+    } else {
+      dot<T>(Nl * l, d2.get(), md1.get(), scal.get());
+      dot<T>(Nl * l, d2.get(), md2.get(), scal.get() + 1);
+    }
+    T t[2];
+    HIP_CHECK(hipMemcpyAsync(t, scal.get(), 2 * sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
+    sync();
+    taug = std::fabs((double)t[0] / (double)t[1]);
+    if (taug < args.taumin) taug = args.taumin;
+    if (taug > args.taumax) taug = args.taumax;
+    std::swap(xq.p, xqp.p);
+    std::swap(xx.p, xxp.p);
+    std::swap(bc.p, bp.p);
+  }
+  // Qbar^T := xq+^T Qs^T  (:359-362)
+  gemm<T>('T', 'N', l, m, Nl, T(1), xqp.get(), Nl, yst, ldyst, T(0), qt, ldqt);
+  *iter_out = it;
+  *mu_out = mu;
+}
+
+// Z^T := Qbar^T op(A)  (projectBlockRow, src/lib/libisvd/core/stage/@x@_postprocess.h:154-157)
+template <typename T>
+static void project(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                    T *zt) {
+  const char transa = (ordera == 'C') ? 'N' : 'T';
+  gemm<T>('N', transa, d.l, d.n, d.m, T(1), qt, ldqt, a, lda, T(0), zt, d.l);
+}
+
+// Gramian postprocessing (src/lib/libisvd/core/stage/@x@_postprocess_gramian.c:135-185).
+template <typename T>
+void stage_post_gramian(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                        T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v) {
+  const i64 l = d.l, k = d.k;
+  DBuf<T> zt(l * d.n), w(l * l), uw(l * l);
+  project<T>(d, ordera, a, lda, qt, ldqt, zt.get());
+  // W := Z^T Z (:145); gesvd('O','N') + vSqrt (:149-151)
+  gemm<T>('N', 'T', l, l, d.n, T(1), zt.get(), l, zt.get(), l, T(0), w.get(), l);
+  svd_small<T>(l, w.get(), l, 0, s, l, uw.get(), l, 0, nullptr, 0, 0, 1);
+  hipLaunchKernelGGL(sqrt_inplace_kernel<T>, dim3(1, 1), dim3(256), 0, ctx().stream, (int)l, s);
+  LAUNCH_CHECK();
+  // U^T := W[:, :k]^T Qbar^T (:157-158)
+  if (want_u) gemm<T>('T', 'N', k, d.m, l, T(1), uw.get(), l, qt, ldqt, T(0), ut, ldut);
+  // V^T := (W[:, :k] / S)^T Z^T (:172-174)
+  if (want_v) {
+    hipLaunchKernelGGL(colscale_inv_kernel<T>, dim3(grid_for(l * k)), dim3(256), 0, ctx().stream,
+                       (int)l, (int)k, (const T *)s, uw.get(), l);
+    LAUNCH_CHECK();
+    gemm<T>('T', 'N', k, d.n, l, T(1), uw.get(), l, zt.get(), l, T(0), vt, ldvt);
+  }
+}
+
+// Tall-skinny-QR postprocessing (src/lib/libisvd/core/stage/@x@_postprocess_tall_skinny_qr.c:
+// 184-296) at P = 1: LQ of Z^T (as the QR of Z by TSQR), SVD of L = R^T, then
+//   U = Qbar U_L[:, :k],  V = Q_lq^T V_L[:, :k].
+// The reference forms U from V_L and V from the rows of U_L (:263-285); that agrees with the
+// above only in projector form and only when k == l, and for k < l its V even depends on the
+// sign convention of gesvd, so it is not reproduced.
+template <typename T>
+void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                     T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v) {
+  const i64 l = d.l, k = d.k;
+  ISVD_CHECK(d.n >= l, "tall-skinny QR postprocessing needs n >= l (n = %lld, l = %lld)", d.n, l);
+  DBuf<T> zt(l * d.n), r(l * l), lmat(l * l), ul(l * l), vl(l * l);
+  project<T>(d, ordera, a, lda, qt, ldqt, zt.get());
+  // Z (row-major n x l in zt) = Q_z R  <=>  Z^T = R^T Q_z^T = L Q_lq   (:199-208)
+  tsqr_rowmajor<T>(d.n, l, zt.get(), l, 0, 1, r.get(), l, 0);
+  hipLaunchKernelGGL(transpose_small_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
+                     ctx().stream, (int)l, (const T *)r.get(), l, lmat.get(), l);
+  LAUNCH_CHECK();
+  // gesvd(L): L = U_L S V_L^T  (:252-258)
+  svd_small<T>(l, lmat.get(), l, 0, s, l, ul.get(), l, 0, vl.get(), l, 0, 1);
+  // U^T := U_L[:, :k]^T Qbar^T  (:263-265)
+  if (want_u) gemm<T>('T', 'N', k, d.m, l, T(1), ul.get(), l, qt, ldqt, T(0), ut, ldut);
+  // V^T := V_L[:, :k]^T Q_lq    (:278-284; Q_lq = Q_z^T is zt as an l x n col-major matrix)
+  if (want_v) gemm<T>('T', 'N', k, d.n, l, T(1), vl.get(), l, zt.get(), l, T(0), vt, ldvt);
+}
+
+#define INST(T)                                                                                \
+  template void gaussian_omega<T>(i64, i64, T *, i64, unsigned long long);                    \
+  template void stage_sketch<T>(const Dims &, char, const T *, i64, const T *, i64, T *, i64); \
+  template void stage_orth_gramian<T>(const Dims &, T *, i64);                                \
+  template void stage_orth_tsqr<T>(const Dims &, T *, i64);                                   \
+  template void stage_integrate_kn<T>(const Dims &, const KnArgs &, const T *, i64, T *, i64,  \
+                                      i64 *, double *);                                        \
+  template void stage_integrate_wy<T>(const Dims &, const WyArgs &, const T *, i64, T *, i64,  \
+                                      i64 *, double *);                                        \
+  template void stage_post_gramian<T>(const Dims &, char, const T *, i64, const T *, i64, T *, \
+                                      T *, i64, T *, i64, bool, bool);                         \
+  template void stage_post_tsqr<T>(const Dims &, char, const T *, i64, const T *, i64, T *,    \
+                                   T *, i64, T *, i64, bool, bool);
+INST(double)
+INST(float)
+#undef INST
+
+}  // namespace isvd

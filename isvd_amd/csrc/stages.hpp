@@ -1,0 +1,58 @@
+// Stage implementations on DEVICE pointers (P = 1: the whole matrix lives on this GPU).
+// Layouts are the reference's: yst = Y^T (Nl x m col-major), qt = Qbar^T (l x m),
+// ut = U^T (l x m, first k rows), vt = V^T (l x n, first k rows); A col-major (ordera 'C')
+// or row-major (ordera 'R').
+#pragma once
+
+#include "common.hpp"
+
+namespace isvd {
+
+struct Dims {
+  i64 m, n, k, l, N, Nl;
+};
+
+struct KnArgs {
+  i64 maxit = 256;
+  double tol = 1e-4;
+};
+
+struct WyArgs {
+  i64 maxit = 256;
+  double tol = 1e-3, tau0 = 1.0, taumax = 1e14, taumin = 1e-14;
+  i64 taumaxit = 20;
+  double beta = 0.5, sigma = 1e-4, eta = 0.85;
+};
+
+template <typename T>
+void gaussian_omega(i64 rows, i64 cols, T *omegat, i64 ld, unsigned long long seed);
+
+template <typename T>
+void stage_sketch(const Dims &d, char ordera, const T *a, i64 lda, const T *omegat, i64 ldo,
+                  T *yst, i64 ldyst);
+
+template <typename T>
+void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst);
+
+template <typename T>
+void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst);
+
+// returns iterations in *iter and the final error in *err (host values)
+template <typename T>
+void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldyst, T *qt,
+                        i64 ldqt, i64 *iter, double *err);
+
+template <typename T>
+void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldyst, T *qt,
+                        i64 ldqt, i64 *iter, double *mu);
+
+// want_u / want_v mirror ut_root >= -1 / vt_root >= -1
+template <typename T>
+void stage_post_gramian(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                        T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v);
+
+template <typename T>
+void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                     T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v);
+
+}  // namespace isvd

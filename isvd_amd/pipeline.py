@@ -1,0 +1,141 @@
+"""Array-level conveniences over the C ABI (device tensors or host numpy arrays).
+
+Row-major views of the reference's column-major buffers are used throughout: Y is m x Nl,
+Qbar m x l, U m x l, V n x l, Omega n x Nl (see capi's module docstring).  A is passed either
+row-major m x n (ordera='R', lda = n) or as the row-major n x m array A^T (ordera='C', lda = m).
+Every call goes through libisvd_mi355x.so; nothing here computes.
+"""
+import ctypes
+
+import numpy as np
+
+from . import capi
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _is_torch(x):
+    return hasattr(x, "data_ptr") and not isinstance(x, np.ndarray)
+
+
+def _dtype(x):
+    if _is_torch(x):
+        return np.float64 if str(x.dtype) == "torch.float64" else np.float32
+    return x.dtype.type
+
+
+def _empty_like_kind(ref, shape, dtype):
+    if _is_torch(ref):
+        torch = _torch()
+        tdt = torch.float64 if dtype == np.float64 else torch.float32
+        return torch.empty(shape, dtype=tdt, device=ref.device)
+    return np.empty(shape, dtype=dtype)
+
+
+def _check(fn_name):
+    err = capi.last_error()
+    if err:
+        raise RuntimeError(f"{fn_name}: {err}")
+
+
+def _a_ld(a, ordera):
+    # ordera 'R': a is row-major m x n (lda = n); 'C': a is A^T row-major n x m (lda = m)
+    return a.shape[1]
+
+
+def sketch(param, a, omega=None, seed=None, ordera="R", dista="R"):
+    """Y = A Omega.  With ``omega`` (n x Nl row-major) uses the caller's draw, else ``seed``."""
+    lib = capi.load()
+    dt = _dtype(a)
+    x = capi.prefix(dt)
+    m, Nl = param.nrow, param.dim_sketch_total
+    y = _empty_like_kind(a, (m, Nl), dt)
+    if omega is not None:
+        getattr(lib, f"isvd_{x}SketchGaussianProjectionOmega")(
+            param, capi._chr(dista), capi._chr(ordera), capi.addr(a), _a_ld(a, ordera),
+            capi.addr(omega), omega.shape[1], capi.addr(y), Nl)
+        _check("sketch")
+    else:
+        getattr(lib, f"isvd_{x}SketchGaussianProjection")(
+            param, None, 0, None, 0, capi._chr(dista), capi._chr(ordera), capi.addr(a),
+            _a_ld(a, ordera), capi.addr(y), Nl, int(seed or 0), 0)
+        _check("sketch")
+    return y
+
+
+def orthogonalize(param, y, alg="GR"):
+    """In-place orthogonalization of every sketch block of Y (m x Nl)."""
+    lib = capi.load()
+    x = capi.prefix(_dtype(y))
+    name = {"GR": "OrthogonalizeGramian", "TS": "OrthogonalizeTallSkinnyQr"}[alg.upper()]
+    getattr(lib, f"isvd_{x}{name}")(param, None, 0, None, 0, capi.addr(y), y.shape[1])
+    _check(name)
+    return y
+
+
+def integrate(param, qs, alg="KN", argv=None):
+    """Returns (Qbar m x l, iter, err_or_mu)."""
+    lib = capi.load()
+    dt = _dtype(qs)
+    x = capi.prefix(dt)
+    ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
+    l = param.dim_sketch
+    qbar = _empty_like_kind(qs, (param.nrow, l), dt)
+    argv = list(argv or [])
+    av = (ct * max(len(argv), 1))(*argv)
+    rv = (ct * 2)()
+    name = {"KN": "IntegrateKolmogorovNagumo", "WY": "IntegrateWenYin"}[alg.upper()]
+    getattr(lib, f"isvd_{x}{name}")(param, ctypes.addressof(av), len(argv),
+                                    ctypes.addressof(rv), 2, capi.addr(qs), qs.shape[1],
+                                    capi.addr(qbar), l)
+    _check(name)
+    return qbar, int(rv[0]), float(rv[1])
+
+
+def postprocess(param, a, qbar, alg="GR", ordera="R", dista="R", want_u=True, want_v=True):
+    """Returns (s[l], U m x l or None, V n x l or None); the first k columns of U/V are valid."""
+    lib = capi.load()
+    dt = _dtype(a)
+    x = capi.prefix(dt)
+    l = param.dim_sketch
+    s = _empty_like_kind(a, (l,), dt)
+    u = _empty_like_kind(a, (param.nrow, l), dt) if want_u else None
+    v = _empty_like_kind(a, (param.ncol, l), dt) if want_v else None
+    name = {"GR": "PostprocessGramian", "TS": "PostprocessTallSkinnyQr"}[alg.upper()]
+    getattr(lib, f"isvd_{x}{name}")(
+        param, None, 0, None, 0, capi._chr(dista), capi._chr(ordera), capi.addr(a),
+        _a_ld(a, ordera), capi.addr(qbar), qbar.shape[1], capi.addr(s), capi.addr(u), l,
+        capi.addr(v), l, 0 if want_u else -2, 0 if want_v else -2)
+    _check(name)
+    return s, u, v
+
+
+def isvd(param, a, omega=None, seed=0, algs=("GP", "GR", "KN", "GR"), ordera="R", dista="R",
+         argv_i=None, want_u=True, want_v=True):
+    """Full driver (isvd_?Isvd / isvd_?IsvdOmega).  Returns dict(s, u, v, iter, err, time)."""
+    lib = capi.load()
+    dt = _dtype(a)
+    x = capi.prefix(dt)
+    ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
+    l = param.dim_sketch
+    s = _empty_like_kind(a, (l,), dt)
+    u = _empty_like_kind(a, (param.nrow, l), dt) if want_u else None
+    v = _empty_like_kind(a, (param.ncol, l), dt) if want_v else None
+    pack = capi.ArgPack(ct, [None, None, list(argv_i) if argv_i else None, None])
+    common = [param, *(al.encode() for al in algs), pack.argv, pack.argc, pack.retv, pack.retc,
+              pack.time, None, capi._chr(dista), capi._chr(ordera), capi.addr(a),
+              _a_ld(a, ordera)]
+    tail = [capi.addr(s), capi.addr(u), l, capi.addr(v), l]
+    roots = [0 if want_u else -2, 0 if want_v else -2]
+    if omega is not None:
+        getattr(lib, f"isvd_{x}IsvdOmega")(*common, capi.addr(omega), omega.shape[1], *tail,
+                                            *roots)
+    else:
+        getattr(lib, f"isvd_{x}Isvd")(*common, *tail, int(seed), *roots)
+    _check("isvd")
+    it, err = pack.ret(2)[:2]
+    return {"s": s, "u": u, "v": v, "iter": int(it), "err": float(err),
+            "time": list(pack.time)}

@@ -1,0 +1,156 @@
+"""Pins of the CPU oracle (oracle/isvd_oracle.py) against the reference's golden vectors
+(data/*.mtx of emfomy/isvd, converted by tests/golden/make_fixtures.py) and against what the
+mathematics fixes.  Tolerances are the reference tests' double-precision ones (derr = 1e-8,
+e.g. check/check/libisvd/core/stage/@x@_integrate_kolmogorov_nagumo.cxx:15-16)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import lowrank_matrix, principal_angles
+
+DERR = 1e-8
+
+
+def proj(q):
+    return q @ q.T
+
+
+# ---- sketch --------------------------------------------------------------------------------
+def test_sketch_golden(golden):
+    # check/.../@x@_sketch_gaussian_projection.hxx: Y vs ys_gp.mtx, Omega = o_gp.mtx
+    y = O.sketch_gaussian_projection(golden["a"], golden["o_gp"])
+    assert np.abs(y - golden["ys_gp"]).max() < DERR
+
+
+def test_sketch_bruteforce():
+    rng = np.random.default_rng(0)
+    a, om = rng.standard_normal((5, 4)), rng.standard_normal((4, 6))
+    ref = np.array([[sum(a[i, k] * om[k, j] for k in range(4)) for j in range(6)]
+                    for i in range(5)])
+    assert np.abs(O.sketch_gaussian_projection(a, om) - ref).max() < 1e-14
+
+
+# ---- orthogonalization ---------------------------------------------------------------------
+@pytest.mark.parametrize("fn", [O.orthogonalize_gramian, O.orthogonalize_tall_skinny_qr])
+def test_orth_golden_projector(golden, fn):
+    # check/.../@x@_orthogonalize_gramian.cxx:100-114: sum_i Q_i Q_i^T vs qs.mtx
+    q = fn(golden["ys"], 12)
+    assert np.abs(proj(q) - proj(golden["qs"])).max() < DERR
+
+
+@pytest.mark.parametrize("fn", [O.orthogonalize_gramian, O.orthogonalize_tall_skinny_qr])
+def test_orth_invariants(fn):
+    rng = np.random.default_rng(1)
+    y = rng.standard_normal((300, 40))
+    q = fn(y, 4)
+    for i in range(4):
+        qi, yi = q[:, i * 10:(i + 1) * 10], y[:, i * 10:(i + 1) * 10]
+        assert np.abs(qi.T @ qi - np.eye(10)).max() < 1e-13
+        assert principal_angles(qi, yi).max() < 1e-12
+
+
+def test_orth_gramian_closed_form():
+    # Q_i = Y_i V diag(lambda)^-1/2 with Y_i^T Y_i = V diag(lambda) V^T: equals the left
+    # singular vectors of Y_i up to column signs
+    rng = np.random.default_rng(2)
+    y = rng.standard_normal((200, 8))
+    q = O.orthogonalize_gramian(y, 1)
+    u, _, _ = np.linalg.svd(y, full_matrices=False)
+    assert np.abs(np.abs(np.sum(q * u, axis=0)) - 1).max() < 1e-12
+
+
+def test_orth_tsqr_matches_numpy_qr():
+    rng = np.random.default_rng(3)
+    y = rng.standard_normal((120, 9))
+    q = O.orthogonalize_tall_skinny_qr(y, 1)
+    qn, _ = np.linalg.qr(y)
+    assert np.abs(np.abs(q) - np.abs(qn)).max() < 1e-12  # equal up to column signs
+
+
+# ---- integration ---------------------------------------------------------------------------
+def test_kn_golden(golden):
+    q, it, err = O.integrate_kolmogorov_nagumo(golden["qs"], 12, 256, 1e-4)
+    assert it == 81 == golden["meta"]["_iterations"]["q_kn"]["iter"]
+    assert np.abs(q - golden["q_kn"]).max() < DERR
+    assert err <= 1e-4
+
+
+def test_wy_golden(golden):
+    q, it, mu = O.integrate_wen_yin(golden["qs"], 12, 256, 1e-3)
+    assert it == 21 == golden["meta"]["_iterations"]["q_wy"]["iter"]
+    assert np.abs(q - golden["q_wy"]).max() < DERR
+    assert mu < 1e-6
+
+
+def test_kn_single_sketch_is_identity():
+    # N = 1: B = I, Z = 0, C = I, F = 0 -> Qbar = Q_0 after exactly one iteration
+    rng = np.random.default_rng(4)
+    q0, _ = np.linalg.qr(rng.standard_normal((50, 6)))
+    q, it, err = O.integrate_kolmogorov_nagumo(q0, 1)
+    assert it == 1 and err < 1e-14
+    assert np.abs(q - q0).max() < 1e-14
+
+
+def test_integration_common_subspace():
+    # every sketch spans the same subspace -> Qbar spans it too, Qbar^T Qbar = I
+    rng = np.random.default_rng(5)
+    base, _ = np.linalg.qr(rng.standard_normal((80, 5)))
+    qs = np.hstack([base @ np.linalg.qr(rng.standard_normal((5, 5)))[0] for _ in range(4)])
+    for fn in (O.integrate_kolmogorov_nagumo, O.integrate_wen_yin):
+        q, it, _ = fn(qs, 4)
+        assert principal_angles(q, base).max() < 1e-10
+        assert np.abs(q.T @ q - np.eye(5)).max() < 1e-10
+
+
+def test_kn_maxit_exhaustion_reports_maxit_plus_one(golden):
+    # @x@_integrate_kolmogorov_nagumo.c:238-241
+    _, it, err = O.integrate_kolmogorov_nagumo(golden["qs"], 12, 5, 1e-4)
+    assert it == 6 and err > 1e-4
+
+
+# ---- postprocessing ------------------------------------------------------------------------
+@pytest.mark.parametrize("fn", [O.postprocess_gramian, O.postprocess_tall_skinny_qr])
+def test_post_golden(golden, fn):
+    # check/.../@x@_postprocess_gramian.hxx:270-300 (k = l, projector comparison)
+    s, u, v = fn(golden["a"], golden["q"], 11)
+    assert np.abs(s - golden["s"][:, 0]).max() < DERR
+    assert np.abs(proj(u) - proj(golden["u"])).max() < DERR
+    assert np.abs(proj(v) - proj(golden["v"])).max() < DERR
+
+
+@pytest.mark.parametrize("fn", [O.postprocess_gramian, O.postprocess_tall_skinny_qr])
+def test_post_is_svd_of_projection(fn):
+    # sigma, U, V are the SVD of Qbar Qbar^T A restricted to span(Qbar)
+    a = lowrank_matrix(150, 90, 0.8 ** np.arange(40), seed=6, noise=1e-3)
+    qbar, _ = np.linalg.qr(a @ np.random.default_rng(7).standard_normal((90, 12)))
+    s, u, v = fn(a, qbar, 5)
+    uu, ss, vvt = np.linalg.svd(qbar.T @ a, full_matrices=False)
+    assert np.abs(s - ss).max() < 1e-12 * ss[0]
+    assert principal_angles(u, qbar @ uu[:, :5]).max() < 1e-10
+    assert principal_angles(v, vvt[:5].T).max() < 1e-10
+
+
+def test_post_tsqr_reference_swap_defect():
+    # k < l: the reference's factor usage (@x@_postprocess_tall_skinny_qr.c:263-285) gives a
+    # different subspace; with k == l the projectors agree, which is why its test misses it
+    a = lowrank_matrix(150, 90, 0.8 ** np.arange(40), seed=8)
+    qbar, _ = np.linalg.qr(a @ np.random.default_rng(9).standard_normal((90, 12)))
+    _, u_ok, v_ok = O.postprocess_tall_skinny_qr(a, qbar, 5)
+    _, u_sw, v_sw = O.postprocess_tall_skinny_qr(a, qbar, 5, reference_swap=True)
+    assert principal_angles(u_ok, u_sw).max() > 1e-3
+    assert principal_angles(v_ok, v_sw).max() > 1e-3
+    _, u_ok, v_ok = O.postprocess_tall_skinny_qr(a, qbar, 12)
+    _, u_sw, v_sw = O.postprocess_tall_skinny_qr(a, qbar, 12, reference_swap=True)
+    assert np.abs(proj(u_ok) - proj(u_sw)).max() < 1e-12
+    assert np.abs(proj(v_ok) - proj(v_sw)).max() < 1e-12
+
+
+# ---- end to end ----------------------------------------------------------------------------
+@pytest.mark.parametrize("algs", [("GR", "KN", "GR"), ("TS", "WY", "TS")])
+def test_isvd_recovers_dominant_singular_values(algs):
+    spec = 0.7 ** np.arange(60)
+    a = lowrank_matrix(400, 200, spec, seed=10)
+    om = np.random.default_rng(11).standard_normal((200, 8 * 16))
+    r = O.isvd(a, om, 6, 8, *algs)
+    # the integration tolerance (1e-4 KN, 1e-3 WY) bounds the accuracy of sigma
+    assert np.abs(r["s"][:6] - spec[:6]).max() < 1e-4
