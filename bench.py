@@ -1,0 +1,204 @@
+"""Benchmark: seconds per rank-k iSVD on one MI355X.
+
+Workload (FP64, synthetic): A = U diag(s) V^T + 1e-6 N(0,1), 131072 x 16384, s_i = 1/i for
+i < 2048, U / V seeded orthonormal (CholeskyQR2 of Gaussian blocks on the GPU), Omega ~ N(0,1)
+16384 x 4096; k = 100, p = 28 (l = 128), N = 32; selectors GP / TS / KN / TS.  A and Omega are
+resident in HBM before the timed region; one step is one full decomposition through the C ABI
+(isvd_dIsvdOmega).  Wen-Yin integration is timed separately on the same input.
+
+Prints ONE JSON line (rank 0).  --gpus N > 1 is rejected: this build runs on one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP64_MFMA = 78.6e12   # MI355X dense FP64 matrix peak (FLOP/s)
+PEAK_HBM = 8.0e12          # MI355X HBM3E (B/s)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_problem(torch, m, n, r, seed, noise=1e-6, dtype=None):
+    """A = U diag(1/i) V^T + noise, generated on the GPU (torch; not part of the timed path)."""
+    dtype = dtype or torch.float64
+    g = torch.Generator(device="cuda").manual_seed(seed)
+
+    def orth(rows):
+        x = torch.randn(rows, r, generator=g, device="cuda", dtype=torch.float64)
+        for _ in range(2):  # CholeskyQR2: x := x R^-1 with R^T R = x^T x
+            c = torch.linalg.cholesky(x.T @ x)
+            eye = torch.eye(r, device="cuda", dtype=torch.float64)
+            rinv = torch.linalg.solve_triangular(c.T, eye, upper=True)
+            x = x @ rinv
+        return x
+
+    u = orth(m)
+    v = orth(n)
+    s = 1.0 / torch.arange(1, r + 1, device="cuda", dtype=torch.float64)
+    us = u * s[None, :]
+    a = torch.empty(m, n, device="cuda", dtype=dtype)
+    blk = 16384
+    for r0 in range(0, m, blk):
+        r1 = min(m, r0 + blk)
+        t = us[r0:r1] @ v.T
+        if noise:
+            t.add_(torch.randn(r1 - r0, n, generator=g, device="cuda", dtype=torch.float64),
+                   alpha=noise)
+        a[r0:r1].copy_(t)
+        del t
+    del u, us
+    return a, s, v
+
+
+def cpu_baseline(a_dev, omega_dev, k, p, N, it_full, m_sample):
+    """The CPU oracle (oracle/isvd_oracle.py, numpy + LAPACK) on the first m_sample rows of the
+    same A with the same Omega; each m-linear stage is scaled by m / m_sample and the KN
+    iteration cost by the full-size iteration count."""
+    import numpy as np
+    import oracle as O
+    try:
+        from threadpoolctl import threadpool_info
+        cores = max((i.get("num_threads", 1) for i in threadpool_info()), default=1)
+    except Exception:  # pragma: no cover
+        cores = os.cpu_count()
+    m = a_dev.shape[0]
+    a = a_dev[:m_sample].cpu().numpy()
+    om = omega_dev.cpu().numpy()
+    t0 = time.perf_counter()
+    y = O.sketch_gaussian_projection(a, om)
+    t1 = time.perf_counter()
+    qs = O.orthogonalize_tall_skinny_qr(y, N)
+    t2 = time.perf_counter()
+    qbar, it_s, _ = O.integrate_kolmogorov_nagumo(qs, N)
+    t3 = time.perf_counter()
+    O.postprocess_tall_skinny_qr(a, qbar, k)
+    t4 = time.perf_counter()
+    scale = m / m_sample
+    est = (t1 - t0 + t2 - t1 + t4 - t3) * scale + (t3 - t2) / it_s * it_full * scale
+    return {"value": est, "unit": "s", "cores": cores, "kind": "port",
+            "sample": (f"oracle GP/TS/KN/TS on rows 0..{m_sample} of the same A and Omega "
+                       f"({t4 - t0:.1f} s of CPU work, {it_s} KN iterations); m-linear stages "
+                       f"scaled x{scale:.0f}, KN per-iteration cost x{it_full} iterations"),
+            "stage_seconds_sample": [t1 - t0, t2 - t1, t3 - t2, t4 - t3]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--m", type=int, default=131072)
+    ap.add_argument("--n", type=int, default=16384)
+    ap.add_argument("--k", type=int, default=100)
+    ap.add_argument("--p", type=int, default=28)
+    ap.add_argument("--N", type=int, default=32)
+    ap.add_argument("--no-wy", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-rows", type=int, default=8192)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this iSVD build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from isvd_amd import capi, pipeline
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    m, n, k, p, N = args.m, args.n, args.k, args.p, args.N
+    l, Nl = k + p, (k + p) * N
+    r = min(2048, n)
+    log(f"generating A {m}x{n} (FP64) and Omega {n}x{Nl} on the GPU")
+    a, s_true, _ = make_problem(torch, m, n, r, seed=1234)
+    gen = torch.Generator(device="cuda").manual_seed(99)
+    omega = torch.randn(n, Nl, generator=gen, device="cuda", dtype=torch.float64)
+    torch.cuda.synchronize()
+    prm = capi.create_param(m, n, k, p, N)
+    algs = ("GP", "TS", "KN", "TS")
+
+    def step(alg_i="KN"):
+        return pipeline.isvd(prm, a, omega=omega, algs=("GP", "TS", alg_i, "TS"))
+
+    for i in range(args.warmup):
+        log(f"warmup {i + 1}/{args.warmup}")
+        res = step()
+    torch.cuda.synchronize()
+    times, stage = [], []
+    t_start = time.perf_counter()
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        res = step()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        stage.append(res["time"])
+        log(f"step {i + 1}/{args.steps}: {times[-1]:.4f} s, stages {['%.4f' % x for x in res['time']]}, KN iter {res['iter']}")
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t_start
+    sec = total / args.steps
+    stage_avg = np.mean(np.array(stage), axis=0).tolist()
+
+    # accuracy (size-independent properties): orthonormality, residual, planted spectrum
+    s_est = res["s"][:k].double()
+    u = res["u"][:, :k]
+    v = res["v"][:, :k]
+    sigma_rel_err = float(((s_est - s_true[:k]).abs() / s_true[:k]).max())
+    orth_u = float((u.T @ u - torch.eye(k, device="cuda", dtype=torch.float64)).abs().max())
+    orth_v = float((v.T @ v - torch.eye(k, device="cuda", dtype=torch.float64)).abs().max())
+    resid = float(torch.linalg.norm(a @ v - u * s_est[None, :]) / torch.linalg.norm(s_est))
+
+    wy = None
+    if not args.no_wy:
+        log("Wen-Yin integration run")
+        t0 = time.perf_counter()
+        rw = step("WY")
+        torch.cuda.synchronize()
+        wy = {"seconds": time.perf_counter() - t0, "stage_seconds": rw["time"],
+              "iterations": rw["iter"],
+              "sigma_max_rel_diff_vs_kn": float(((rw["s"][:k] - res["s"][:k]).abs()
+                                                 / res["s"][:k]).max())}
+
+    # dominant kernel: the sketch GEMM (2 m n Nl FLOP per launch), timed with HIP events on the
+    # library's stream by the driver (time[0])
+    flop = 2.0 * m * n * Nl
+    t_sketch = stage_avg[0]
+    achieved = flop / t_sketch
+    roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
+                "kernel": "gemm_kernel<double,N,T,128,128,16> (sketch Y^T = Omega^T A^T)",
+                "flop_per_launch": flop, "avg_launch_seconds": t_sketch}
+    prof = os.path.join(ROOT, "profiles", "sketch_gemm_fp64_131072x16384_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as f:
+            roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
+
+    cpu = None
+    if not args.no_cpu:
+        log("CPU baseline (oracle on a row sample)")
+        cpu = cpu_baseline(a, omega, k, p, N, res["iter"], min(args.cpu_rows, m))
+
+    out = {
+        "metric": "sec per rank-k iSVD & MFMA/HBM % roofline; mixed vs FP64 at stated σ error",
+        "value": sec, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": sec * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"iSVD FP64 A {m}x{n} col-major-equivalent, k={k} p={p} N={N}, "
+                               f"GP/TS/KN/TS, s_i=1/i (i<{r}) + 1e-6 noise",
+                   "m": m, "n": n, "k": k, "p": p, "N": N, "algs": list(algs)},
+        "stage_seconds": stage_avg, "kn_iterations": res["iter"],
+        "accuracy": {"sigma_max_rel_err_vs_planted": sigma_rel_err, "UtU_minus_I": orth_u,
+                     "VtV_minus_I": orth_v, "residual_AV_minus_US_rel": resid},
+        "wen_yin": wy, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -181,6 +181,15 @@ T read_scalar(const T *dptr) {
   return h;
 }
 
+// ISVD_TRACE=1 prints one line per integration iteration (and line-search trip) to stderr.
+bool trace_on() {
+  static const bool on = [] {
+    const char *e = std::getenv("ISVD_TRACE");
+    return e && e[0] == '1';
+  }();
+  return on;
+}
+
 int grid_for(i64 n) {
   i64 b = (n + 255) / 256;
   if (b > 8192) b = 8192;
@@ -282,6 +291,7 @@ void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldy
     gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, cinv.get(), l, T(1), bp.get(), Nl);
     // convergence (:230-241)
     error = (double)read_scalar(err.get());
+    if (trace_on()) std::fprintf(stderr, "isvd KN iter %lld err %.17g\n", it, error);
     if (error <= args.tol) break;
     if (it == args.maxit) {
       ++it;
@@ -354,6 +364,9 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
       gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, fgc, l, T(1), bp.get(), Nl);
       dot<T>(Nl * l, bp.get(), bp.get(), scal.get() + 2);
       phit = (double)half_ninv * (double)read_scalar(scal.get() + 2);
+      if (trace_on())
+        std::fprintf(stderr, "isvd WY iter %lld trip %lld tau %.17g phit %.17g phi %.17g\n", it,
+                     tauiter, tau, phit, phi);
       if (phit >= phi + tau * args.sigma * mu) break;
     }
     // (:287-288)
@@ -372,6 +385,7 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
                        (const T *)dgc.get(), (const T *)dc.get(), scal.get());
     LAUNCH_CHECK();
     mu = (double)read_scalar(scal.get());
+    if (trace_on()) std::fprintf(stderr, "isvd WY iter %lld mu %.17g\n", it, mu);
     // convergence (:312-319)
     if (mu < args.tol * args.tol) break;
     if (it == args.maxit) {

@@ -60,7 +60,8 @@ def test_orthogonalize_golden(golden, torch_cuda, dtype, alg):
     assert np.abs(proj(q) - proj(golden["qs"])).max() < TOL[dtype]
     # the Gramian method squares cond(Y_i): its orthogonality is bounded by the oracle's own
     # loss of orthogonality on the same input (TS is orthogonal to rounding)
-    q_ref = (O.orthogonalize_gramian if alg == "GR" else O.orthogonalize_tall_skinny_qr)(ys, 12)
+    q_ref = (O.orthogonalize_gramian if alg == "GR" else O.orthogonalize_tall_skinny_qr)(
+        ys.astype(dtype), 12).astype(np.float64)
     for i in range(12):
         qi, ri = q[:, 11 * i:11 * (i + 1)], q_ref[:, 11 * i:11 * (i + 1)]
         ref_err = np.abs(ri.T @ ri - np.eye(11)).max()
