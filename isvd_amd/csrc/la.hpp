@@ -49,6 +49,25 @@ void syev_small(i64 n, const T *A, i64 lda, T *w, T *Z, i64 ldz);
 template <typename T>
 void tsqr_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr, i64 sR);
 
+// l x l products (l <= 128): C := alpha op(A) op(B) + beta C + diag I; one short launch.
+template <typename T>
+void small_gemm(char ta, char tb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda, const T *B,
+                i64 ldb, T beta, T *C, i64 ldc, T diag = T(0));
+
+// C = (I/2 + (I/4 - Z)^{1/2})^{1/2} and C^-1 (n x n, ld n) by coupled Newton-Schulz square
+// roots; valid while ||4 Z||_2 < 1.  work: 6 n*n entries.
+template <typename T>
+void kn_matfun(i64 n, const T *Z, T *C, T *Cinv, T *work, int iters1, int iters2);
+
+// out[0] = ||A - c I||_F  (n x n, ld n)
+template <typename T>
+void frob_shift(i64 n, const T *A, T c, T *out);
+
+// In-place inverse (Gauss-Jordan, partial pivoting), n <= 128; *info_dev = 0 or the 1-based
+// index of a zero pivot.
+template <typename T>
+void inv_small(i64 n, T *A, i64 lda, int *info_dev);
+
 // Reductions used by the integration loops (results left on device).
 template <typename T>
 void dot(i64 n, const T *x, const T *y, T *result);  // result[0] = sum x*y (deterministic)

@@ -105,15 +105,14 @@ __global__ void sym_from_upper_kernel(int n, T *A, i64 lda) {
 // (@x@_integrate_wen_yin.c:249-259), eliminating with the C_12 = I/2 block gives
 //   Fgc = [inv(C)_12 Dc - inv(C)_11] = -(2/tau) S^-1,
 //   Fc  = [inv(C)_22 Dc - inv(C)_21] + I = (2/tau) (Dc - (2/tau) I) S^-1 - I,
-// with S = -(Dgc - Dc^2)/2 - (2/tau^2) I (C is singular exactly when S is).  S^-1 comes from
-// one eigendecomposition Dgc - Dc^2 = V diag(w) V^T per iteration: S^-1 = V diag(mu) V^T,
-// mu_i = 1 / (-w_i/2 - 2/tau^2), so each line-search trip costs l x l products only.
+// with S = -(Dgc - Dc^2)/2 - (2/tau^2) I, which is singular exactly when C is.  S^-1 is an
+// l x l Gauss-Jordan inverse (the reference inverts the 2l x 2l C with getrf + getri).
+// out := a * sym(A) + b * I
 template <typename T>
-__global__ void wy_vscale_kernel(int l, const T *w, const T *V, T inv_tau, T *VS) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < l * l; e += gridDim.x * blockDim.x) {
-    const int j = e / l;
-    const T mu = T(1) / (T(-0.5) * w[j] - T(2) * inv_tau * inv_tau);
-    VS[e] = V[e] * mu;
+__global__ void sym_scale_kernel(int n, const T *A, T *out, T a, T b) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
+    const int i = e % n, j = e / n;
+    out[e] = a * T(0.5) * (A[i + j * n] + A[j + i * n]) + ((i == j) ? b : T(0));
   }
 }
 
@@ -263,35 +262,57 @@ void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldy
   omatcopy<T>('N', Nl, l, T(1), M.get(), Nl, bc.get(), Nl);
   i64 it;
   double error = -INFINITY;
+  DBuf<T> work(6 * l * l), rho(1);
   for (it = 1;; ++it) {
     // Bgc := M Bc / N  (:174-178)
     gemm<T>('N', 'N', Nl, l, Nl, ninv, M.get(), Nl, bc.get(), Nl, T(0), bgc.get(), Nl);
     // Dc := Bc^T Bc / N (:181); Z := Bc^T Bgc / N - Dc^T Dc (:184-190)
     gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bc.get(), Nl, T(0), dc.get(), l);
     gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bgc.get(), Nl, T(0), z.get(), l);
-    gemm<T>('T', 'N', l, l, l, T(-1), dc.get(), l, dc.get(), l, T(1), z.get(), l);
-    // eig(Z) (:193), scalars (:196-206)
-    syev_small<T>(l, z.get(), l, w.get(), zv.get(), l);
-    hipLaunchKernelGGL(kn_scalars_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
-                       (const T *)w.get(), (const T *)zv.get(), zs.get(), zi.get(), err.get());
-    LAUNCH_CHECK();
-    // C := ZS ZS^T (:209); inv(C) := ZI ZI^T, upper (:212)
-    gemm<T>('N', 'T', l, l, l, T(1), zs.get(), l, zs.get(), l, T(0), c.get(), l);
-    gemm<T>('N', 'T', l, l, l, T(1), zi.get(), l, zi.get(), l, T(0), cinv.get(), l);
-    hipLaunchKernelGGL(sym_from_upper_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
-                       ctx().stream, (int)l, cinv.get(), l);
-    LAUNCH_CHECK();
-    // Fc [in C] := C - Dc inv(C)  (:218)
-    gemm<T>('N', 'N', l, l, l, T(-1), dc.get(), l, cinv.get(), l, T(1), c.get(), l);
-    // X_+ := X_c Fc + (Bc / N) inv(C)  (image of :221-222)
-    gemm<T>('N', 'N', Nl, l, l, T(1), xc.get(), Nl, c.get(), l, T(0), xp.get(), Nl);
-    gemm<T>('N', 'N', Nl, l, l, ninv, bc.get(), Nl, cinv.get(), l, T(1), xp.get(), Nl);
-    // B_+ := Bc Fc + Bgc inv(C)  (:225-226)
-    gemm<T>('N', 'N', Nl, l, l, T(1), bc.get(), Nl, c.get(), l, T(0), bp.get(), Nl);
-    gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, cinv.get(), l, T(1), bp.get(), Nl);
+    small_gemm<T>('T', 'N', l, l, l, T(-1), dc.get(), l, dc.get(), l, T(1), z.get(), l);
+    // C and inv(C) (:193-212): C = (I/2 + (I/4 - Z)^{1/2})^{1/2} = Z diag(s) Z^T with
+    // s = sqrt(1/2 + sqrt(1/4 - eig(Z))).  Newton-Schulz square roots while ||4 Z|| < 0.6
+    // (checked below from ||Z||_F), the eigendecomposition otherwise.
+    bool use_eig = false;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      if (!use_eig) {
+        frob_shift<T>(l, z.get(), T(0), rho.get());
+        kn_matfun<T>(l, z.get(), c.get(), cinv.get(), work.get(), 7, 6);
+        frob_shift<T>(l, c.get(), T(1), err.get());  // ||I - C||_F = ||s - 1||_2
+      } else {
+        syev_small<T>(l, z.get(), l, w.get(), zv.get(), l);
+        hipLaunchKernelGGL(kn_scalars_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
+                           (const T *)w.get(), (const T *)zv.get(), zs.get(), zi.get(),
+                           err.get());
+        LAUNCH_CHECK();
+        small_gemm<T>('N', 'T', l, l, l, T(1), zs.get(), l, zs.get(), l, T(0), c.get(), l);
+        small_gemm<T>('N', 'T', l, l, l, T(1), zi.get(), l, zi.get(), l, T(0), cinv.get(), l);
+        hipLaunchKernelGGL(sym_from_upper_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
+                           ctx().stream, (int)l, cinv.get(), l);
+        LAUNCH_CHECK();
+      }
+      // Fc [in C] := C - Dc inv(C)  (:218)
+      small_gemm<T>('N', 'N', l, l, l, T(-1), dc.get(), l, cinv.get(), l, T(1), c.get(), l);
+      // X_+ := X_c Fc + (Bc / N) inv(C)  (image of :221-222)
+      gemm<T>('N', 'N', Nl, l, l, T(1), xc.get(), Nl, c.get(), l, T(0), xp.get(), Nl);
+      gemm<T>('N', 'N', Nl, l, l, ninv, bc.get(), Nl, cinv.get(), l, T(1), xp.get(), Nl);
+      // B_+ := Bc Fc + Bgc inv(C)  (:225-226)
+      gemm<T>('N', 'N', Nl, l, l, T(1), bc.get(), Nl, c.get(), l, T(0), bp.get(), Nl);
+      gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, cinv.get(), l, T(1), bp.get(), Nl);
+      T h[2];
+      HIP_CHECK(hipMemcpyAsync(&h[0], err.get(), sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
+      if (!use_eig)
+        HIP_CHECK(hipMemcpyAsync(&h[1], rho.get(), sizeof(T), hipMemcpyDeviceToHost,
+                                 ctx().stream));
+      sync();
+      error = (double)h[0];
+      if (use_eig || 4.0 * (double)h[1] < 0.6) break;
+      use_eig = true;  // Newton-Schulz not guaranteed to converge: redo with eig(Z)
+    }
+    if (trace_on())
+      std::fprintf(stderr, "isvd KN iter %lld err %.17g (%s)\n", it, error,
+                   use_eig ? "eig" : "newton-schulz");
     // convergence (:230-241)
-    error = (double)read_scalar(err.get());
-    if (trace_on()) std::fprintf(stderr, "isvd KN iter %lld err %.17g\n", it, error);
     if (error <= args.tol) break;
     if (it == args.maxit) {
       ++it;
@@ -308,7 +329,7 @@ void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldy
 
 // Wen-Yin integration (@x@_integrate_wen_yin.c:201-384), same coordinates as KN:
 // Q_c = Qs xq, G_c = Qs gq (gq = B_c / N), X_c = Qs xx, <Qs a, Qs b> = <a, M b>; the
-// 2l x 2l inverse of the line search is replaced by its closed form (wy_vscale_kernel).
+// 2l x 2l inverse of the line search is replaced by its closed form (sym_scale_kernel).
 template <typename T>
 void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldyst, T *qt,
                         i64 ldqt, i64 *iter_out, double *mu_out) {
@@ -317,8 +338,8 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
   const T half_ninv = T(0.5) / T(d.N);
   DBuf<T> M(Nl * Nl), xq(Nl * l), xqp(Nl * l), xx(Nl * l), xxp(Nl * l), bc(Nl * l), bp(Nl * l),
       bgc(Nl * l), gq(Nl * l), d1(Nl * l), d2(Nl * l), md1(Nl * l), md2(Nl * l);
-  DBuf<T> dc(l * l), dgc(l * l), zm(l * l), w(l), zv(l * l), vs(l * l), si(l * l), fcb(l * l),
-      fgcb(l * l), scal(4);
+  DBuf<T> dc(l * l), dgc(l * l), zm(l * l), si(l * l), fcb(l * l), fgcb(l * l), scal(4);
+  DBuf<int> info(1);
   gemm<T>('N', 'T', Nl, Nl, m, T(1), yst, ldyst, yst, ldyst, T(0), M.get(), Nl);
   // init (:201-235)
   fill<T>(Nl, l, T(0), xq.get(), Nl, T(1));
@@ -340,30 +361,35 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
   T *fgc = fgcb.get();
   i64 it;
   for (it = 1;; ++it) {
-    // step size (:243-282); eig(Dgc - Dc^2) once per iteration, then closed-form trips
-    gemm<T>('T', 'N', l, l, l, T(-1), dc.get(), l, dc.get(), l, T(0), zm.get(), l);
+    // step size (:243-282); Z = Dgc - Dc^2 once per iteration, then per trip the closed form
+    small_gemm<T>('T', 'N', l, l, l, T(-1), dc.get(), l, dc.get(), l, T(0), zm.get(), l);
     hipLaunchKernelGGL(axpy_diag_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
                        (int)l, zm.get(), (const T *)dgc.get(), T(1), T(0));
     LAUNCH_CHECK();
-    syev_small<T>(l, zm.get(), l, w.get(), zv.get(), l);
     double tau = taug, phit = phi;
     for (i64 tauiter = 1; tauiter <= args.taumaxit; ++tauiter, tau *= args.beta) {
       const T it_ = (T)(1.0 / tau);
-      hipLaunchKernelGGL(wy_vscale_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
-                         (int)l, (const T *)w.get(), (const T *)zv.get(), it_, vs.get());
+      // S = -Z/2 - (2/tau^2) I ; S^-1 ; Fgc = -(2/tau) S^-1 ; Fc = (2/tau) Dc S^-1 - (4/tau^2) S^-1 - I
+      hipLaunchKernelGGL(sym_scale_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
+                         (int)l, (const T *)zm.get(), si.get(), T(-0.5), T(-2) * it_ * it_);
       LAUNCH_CHECK();
-      // S^-1 = V diag(mu) V^T ; Fgc = -(2/tau) S^-1 ; Fc = (2/tau) Dc S^-1 - (4/tau^2) S^-1 - I
-      gemm<T>('N', 'T', l, l, l, T(1), vs.get(), l, zv.get(), l, T(0), si.get(), l);
-      gemm<T>('N', 'T', l, l, l, T(-2) * it_, vs.get(), l, zv.get(), l, T(0), fgc, l);
-      gemm<T>('N', 'N', l, l, l, T(2) * it_, dc.get(), l, si.get(), l, T(0), fc, l);
+      inv_small<T>(l, si.get(), l, info.get());
+      hipLaunchKernelGGL(sym_scale_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
+                         (int)l, (const T *)si.get(), fgc, T(-2) * it_, T(0));
+      LAUNCH_CHECK();
+      small_gemm<T>('N', 'N', l, l, l, T(2) * it_, dc.get(), l, si.get(), l, T(0), fc, l, T(-1));
       hipLaunchKernelGGL(axpy_diag_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
-                         (int)l, fc, (const T *)si.get(), T(-4) * it_ * it_, T(-1));
+                         (int)l, fc, (const T *)si.get(), T(-4) * it_ * it_, T(0));
       LAUNCH_CHECK();
       // B+ := Bc Fc + Bgc Fgc  (:270-271)
       gemm<T>('N', 'N', Nl, l, l, T(1), bc.get(), Nl, fc, l, T(0), bp.get(), Nl);
       gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, fgc, l, T(1), bp.get(), Nl);
       dot<T>(Nl * l, bp.get(), bp.get(), scal.get() + 2);
+      int h_info = 0;
+      HIP_CHECK(hipMemcpyAsync(&h_info, info.get(), sizeof(int), hipMemcpyDeviceToHost,
+                               ctx().stream));
       phit = (double)half_ninv * (double)read_scalar(scal.get() + 2);
+      ISVD_CHECK(h_info == 0, "Wen-Yin step matrix is singular (pivot %d)", h_info);
       if (trace_on())
         std::fprintf(stderr, "isvd WY iter %lld trip %lld tau %.17g phit %.17g phi %.17g\n", it,
                      tauiter, tau, phit, phi);

@@ -6,8 +6,8 @@
 //  src/lib/libisvd/core/stage/@x@_postprocess_tall_skinny_qr.c:184-246).  On one GPU the same
 // tree runs over row panels of each sketch:
 //   level 0   : every row panel (<= 256 rows) is factored by one workgroup with the panel held
-//               in registers (column c = tid % LC, rows rg + NRG*i), Householder vectors
-//               written back in place, R written to the next level's buffer;
+//               in registers (each wave owns whole columns, each lane four rows), Householder
+//               vectors written back in place, R written to the next level's buffer;
 //   level k>0 : two consecutive R factors form a stacked 2l x l panel that is factored the
 //               same way, until one R per sketch remains;
 //   formation : top-down, each node applies its reflectors to [C; 0] (C = I at the root) and
@@ -19,9 +19,15 @@ namespace isvd {
 
 namespace {
 
-constexpr int kNT = 512;    // threads per panel (2 waves per SIMD -> 256 VGPRs per lane)
-constexpr int kMaxR = 64;   // rows per thread
-constexpr int kHMax = 256;  // rows per panel (LDS broadcast buffer size)
+// Panel layout: 8 waves; wave w owns columns [w*cpw, (w+1)*cpw) of the panel (cpw = LC / 8,
+// LC = l rounded up to a power of two >= 8), lane owns rows lane + 64 i (i < 4).  Column norms
+// and the products v^T A[:, c] are wave reductions (shuffles); the Householder vector of
+// each step is broadcast through a double-buffered LDS vector, one barrier per step.
+constexpr int kWaves = 8;
+constexpr int kNT = 64 * kWaves;  // threads per panel
+constexpr int kRPL = 4;           // rows per lane
+constexpr int kMaxCPW = 16;       // columns per wave (l <= 128)
+constexpr int kHMax = 64 * kRPL;  // rows per panel
 
 struct PanelGeom {
   i64 m;        // rows of the level-0 matrix
@@ -42,7 +48,6 @@ struct LevelArgs {
   int l, LC;
   bool leaf;
   PanelGeom geom;
-  // panel storage (in/out for factor; V source for apply)
   void *Y; i64 ldy, sY;
   void *Rin; int nchild;    // internal: stacked children factors
   int nnode;                // nodes at this level per batch entry
@@ -70,168 +75,182 @@ __device__ __forceinline__ void panel_of(const LevelArgs &a, int b, int j, T *&b
 }
 
 template <typename T>
-__global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
-  __shared__ T vsh[kHMax];
-  __shared__ T wpart[kNT];
-  __shared__ T red[kNT];
-  __shared__ T alpha_sh;
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
 
+template <typename T>
+__global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
+  __shared__ T vsh[2][kHMax];
+  __shared__ T tau_sh[2];
   const int j_node = blockIdx.x, b = blockIdx.y;
-  const int l = a.l, LC = a.LC, NRG = kNT / LC;
-  const int tid = threadIdx.x, c = tid % LC, rg = tid / LC;
+  const int l = a.l, cpw = a.LC / kWaves;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *base;
   i64 ld;
   int h;
   panel_of<T>(a, b, j_node, base, ld, h);
+  for (int e = tid; e < 2 * kHMax; e += kNT) (&vsh[0][0])[e] = T(0);
 
-  // rows of this thread: r = rg + NRG * i for i in [0, ihi); "rows >= j" is i >= first_i(j)
-  const int ihi = (h > rg) ? (h - rg + NRG - 1) / NRG : 0;
-  auto first_i = [&](int j) { return (j <= rg) ? 0 : (j - rg + NRG - 1) / NRG; };
-  T x[kMaxR];
+  T x[kMaxCPW][kRPL];
 #pragma unroll
-  for (int i = 0; i < kMaxR; ++i) {
-    const int r = rg + NRG * i;
-    x[i] = (i < ihi && c < l) ? base[(i64)r * ld + c] : T(0);
-  }
+  for (int cc = 0; cc < kMaxCPW; ++cc)
+#pragma unroll
+    for (int i = 0; i < kRPL; ++i) {
+      const int r = lane + 64 * i, c = wave * cpw + cc;
+      x[cc][i] = (cc < cpw && r < h && c < l) ? base[(i64)r * ld + c] : T(0);
+    }
   const int kmax = h < l ? h : l;
   T *tau_out = reinterpret_cast<T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
+  __syncthreads();
 
   for (int j = 0; j < kmax; ++j) {
-    const int i_ge = first_i(j), i_gt = first_i(j + 1);
-    const bool own_diag = (rg == j % NRG);
-    const int i_diag = j / NRG;
-    // (1) norm of column j below the diagonal, and the diagonal entry
-    if (c == j) {
-      T part = 0;
+    const int buf = j & 1;
+    if (wave == j / cpw) {
+      const int jj = j % cpw;
+      T col[kRPL];
 #pragma unroll
-      for (int i = 0; i < kMaxR; ++i) {
-        if (i >= i_gt && i < ihi) part += x[i] * x[i];
-        if (own_diag && i == i_diag) alpha_sh = x[i];
+      for (int i = 0; i < kRPL; ++i) col[i] = T(0);
+#pragma unroll
+      for (int cc = 0; cc < kMaxCPW; ++cc)
+        if (cc == jj)
+#pragma unroll
+          for (int i = 0; i < kRPL; ++i) col[i] = x[cc][i];
+      T part = 0, av = 0;
+#pragma unroll
+      for (int i = 0; i < kRPL; ++i) {
+        const int r = lane + 64 * i;
+        if (r > j) part += col[i] * col[i];  // rows >= h hold zeros
+        if (r == j) av = col[i];
       }
-      red[rg] = part;
-    }
-    __syncthreads();
-    T sigma2 = 0;
-    for (int q = 0; q < NRG; ++q) sigma2 += red[q];
-    const T alpha = alpha_sh;
-    T beta, tau, scale;
-    if (sigma2 == T(0)) {
-      beta = alpha; tau = T(0); scale = T(0);
-    } else {
-      const T nrm = sqrt(alpha * alpha + sigma2);
-      beta = (alpha >= T(0)) ? -nrm : nrm;
-      tau = (beta - alpha) / beta;
-      scale = T(1) / (alpha - beta);
-    }
-    // (2) form v in column j, broadcast through LDS
-    if (c == j) {
-#pragma unroll
-      for (int i = 0; i < kMaxR; ++i) {
-        const int r = rg + NRG * i;
-        if (i >= i_gt && i < ihi) { x[i] *= scale; vsh[r] = x[i]; }
-        if (own_diag && i == i_diag) { x[i] = beta; vsh[r] = T(1); }
+      const T sigma2 = wave_sum(part);
+      const T alpha = wave_sum(av);  // one lane contributes
+      T beta, tau, scale;
+      if (sigma2 == T(0)) {
+        beta = alpha; tau = T(0); scale = T(0);
+      } else {
+        const T nrm = sqrt(alpha * alpha + sigma2);
+        beta = (alpha >= T(0)) ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        scale = T(1) / (alpha - beta);
       }
-      if (tid == j) tau_out[j] = tau;  // thread (c == j, rg == 0)
+#pragma unroll
+      for (int i = 0; i < kRPL; ++i) {
+        const int r = lane + 64 * i;
+        if (r > j) col[i] *= scale;
+        else if (r == j) col[i] = beta;
+        if (r < kHMax) vsh[buf][r] = (r > j && r < h) ? col[i] : (r == j ? T(1) : T(0));
+      }
+#pragma unroll
+      for (int cc = 0; cc < kMaxCPW; ++cc)
+        if (cc == jj)
+#pragma unroll
+          for (int i = 0; i < kRPL; ++i) x[cc][i] = col[i];
+      if (lane == 0) {
+        tau_out[j] = tau;
+        tau_sh[buf] = tau;
+      }
     }
     __syncthreads();
-    // (3) w_c = v^T A[:, c] for c > j
-    if (c > j && c < l) {
-      T part = 0;
+    const T tau = tau_sh[buf];
+    if (tau != T(0)) {
+      T v[kRPL];
 #pragma unroll
-      for (int i = 0; i < kMaxR; ++i)
-        if (i >= i_ge && i < ihi) part += vsh[rg + NRG * i] * x[i];
-      wpart[rg * LC + c] = part;
-    }
-    __syncthreads();
-    // (4) rank-1 update of the trailing columns
-    if (c > j && c < l && tau != T(0)) {
-      T w = 0;
-      for (int q = 0; q < NRG; ++q) w += wpart[q * LC + c];
-      w *= tau;
+      for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][lane + 64 * i];
 #pragma unroll
-      for (int i = 0; i < kMaxR; ++i)
-        if (i >= i_ge && i < ihi) x[i] -= vsh[rg + NRG * i] * w;
+      for (int cc = 0; cc < kMaxCPW; ++cc) {
+        const int c = wave * cpw + cc;
+        if (cc < cpw && c > j && c < l) {
+          T part = 0;
+#pragma unroll
+          for (int i = 0; i < kRPL; ++i) part += v[i] * x[cc][i];
+          const T w = wave_sum(part) * tau;
+#pragma unroll
+          for (int i = 0; i < kRPL; ++i) x[cc][i] -= v[i] * w;
+        }
+      }
     }
   }
-  // write V (below the diagonal) + R (on and above) back in place; R to the next level
-#pragma unroll
-  for (int i = 0; i < kMaxR; ++i) {
-    const int r = rg + NRG * i;
-    if (i < ihi && c < l) base[(i64)r * ld + c] = x[i];
-  }
+  // V (below the diagonal) and R (on and above) back in place; R to the next level
   T *R = reinterpret_cast<T *>(a.Rout) + ((i64)b * a.nnode + j_node) * l * l;
 #pragma unroll
-  for (int i = 0; i < kMaxR; ++i) {
-    const int r = rg + NRG * i;
-    if (r < l && c < l) R[r * l + c] = (r <= c && r < kmax) ? x[i] : T(0);
-  }
-  // rows of R beyond the panel height (h < l) are zero
-  for (int e = tid; e < l * l; e += kNT) {
-    const int r = e / l;
-    if (r >= h) R[e] = T(0);
-  }
+  for (int cc = 0; cc < kMaxCPW; ++cc)
+#pragma unroll
+    for (int i = 0; i < kRPL; ++i) {
+      const int r = lane + 64 * i, c = wave * cpw + cc;
+      if (cc < cpw && c < l) {
+        if (r < h) base[(i64)r * ld + c] = x[cc][i];
+        if (r < l) R[r * l + c] = (r <= c && r < kmax) ? x[cc][i] : T(0);
+      }
+    }
 }
 
-// Explicit Q: X := H_0 H_1 ... H_{k-1} [C; 0].  The reflectors are applied last-to-first; the
-// vector of step j is streamed from the panel into LDS one step ahead (every lane loads one
-// entry), so only X lives in registers.
+// Explicit Q: X := H_0 H_1 ... H_{k-1} [C; 0], reflectors applied last-to-first.  The vector of
+// step j - 1 is loaded from the panel by one wave while step j computes.
 template <typename T>
 __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
   __shared__ T vsh[2][kHMax];
-  __shared__ T wpart[kNT];
-
   const int j_node = blockIdx.x, b = blockIdx.y;
-  const int l = a.l, LC = a.LC, NRG = kNT / LC;
-  const int tid = threadIdx.x, c = tid % LC, rg = tid / LC;
+  const int l = a.l, cpw = a.LC / kWaves;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
   T *base;
   i64 ld;
   int h;
   panel_of<T>(a, b, j_node, base, ld, h);
   const int kmax = h < l ? h : l;
-
-  T x[kMaxR];
   const T *C = reinterpret_cast<const T *>(a.Cin) + ((i64)b * a.nnode + j_node) * l * l;
-#pragma unroll
-  for (int i = 0; i < kMaxR; ++i) {
-    const int r = rg + NRG * i;
-    x[i] = (r < kmax && c < l) ? C[r * l + c] : T(0);
-  }
   const T *tau_in = reinterpret_cast<const T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
-  // v_j[r] = 0 (r < j), 1 (r == j), V[r, j] (r > j)
+  T x[kMaxCPW][kRPL];
+#pragma unroll
+  for (int cc = 0; cc < kMaxCPW; ++cc)
+#pragma unroll
+    for (int i = 0; i < kRPL; ++i) {
+      const int r = lane + 64 * i, c = wave * cpw + cc;
+      x[cc][i] = (cc < cpw && r < kmax && c < l) ? C[r * l + c] : T(0);
+    }
+  // v_j[r] = 0 (r < j), 1 (r == j), V[r, j] (j < r < h), 0 (r >= h)
   auto vload = [&](int j, int r) -> T {
-    return (r < j) ? T(0) : (r == j ? T(1) : base[(i64)r * ld + j]);
+    return (r < j || r >= h) ? T(0) : (r == j ? T(1) : base[(i64)r * ld + j]);
   };
-  if (kmax > 0 && tid < h) vsh[(kmax - 1) & 1][tid] = vload(kmax - 1, tid);
+  if (kmax > 0 && wave == 0) {
+#pragma unroll
+    for (int i = 0; i < kRPL; ++i) vsh[(kmax - 1) & 1][lane + 64 * i] = vload(kmax - 1, lane + 64 * i);
+  }
   __syncthreads();
   for (int j = kmax - 1; j >= 0; --j) {
-    const T *vs = vsh[j & 1];
-    T vnext = T(0);
-    if (j > 0 && tid < h) vnext = vload(j - 1, tid);  // in flight during this step
+    const int buf = j & 1;
+    const bool loader = (j > 0) && (wave == (j - 1) % kWaves);
+    T vnext[kRPL];
+    if (loader) {
+#pragma unroll
+      for (int i = 0; i < kRPL; ++i) vnext[i] = vload(j - 1, lane + 64 * i);
+    }
     const T tau = tau_in[j];
-    if (c < l) {
-      T part = 0;
+    if (tau != T(0)) {
+      T v[kRPL];
 #pragma unroll
-      for (int i = 0; i < kMaxR; ++i) {
-        const int r = rg + NRG * i;
-        if (r >= j && r < h) part += vs[r] * x[i];
-      }
-      wpart[rg * LC + c] = part;
-    }
-    __syncthreads();
-    if (c < l && tau != T(0)) {
-      T w = 0;
-      for (int q = 0; q < NRG; ++q) w += wpart[q * LC + c];
-      w *= tau;
+      for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][lane + 64 * i];
 #pragma unroll
-      for (int i = 0; i < kMaxR; ++i) {
-        const int r = rg + NRG * i;
-        if (r >= j && r < h) x[i] -= vs[r] * w;
+      for (int cc = 0; cc < kMaxCPW; ++cc) {
+        const int c = wave * cpw + cc;
+        if (cc < cpw && c < l) {
+          T part = 0;
+#pragma unroll
+          for (int i = 0; i < kRPL; ++i) part += v[i] * x[cc][i];
+          const T w = wave_sum(part) * tau;
+#pragma unroll
+          for (int i = 0; i < kRPL; ++i) x[cc][i] -= v[i] * w;
+        }
       }
     }
-    if (j > 0 && tid < h) vsh[(j - 1) & 1][tid] = vnext;
+    if (loader) {
+#pragma unroll
+      for (int i = 0; i < kRPL; ++i) vsh[buf ^ 1][lane + 64 * i] = vnext[i];
+    }
     __syncthreads();
   }
-  // output: leaf -> the panel itself; internal -> children C blocks (same layout as Rin)
   T *out;
   i64 ldo;
   if (a.leaf) {
@@ -241,10 +260,12 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
     ldo = l;
   }
 #pragma unroll
-  for (int i = 0; i < kMaxR; ++i) {
-    const int r = rg + NRG * i;
-    if (r < h && c < l) out[(i64)r * ldo + c] = x[i];
-  }
+  for (int cc = 0; cc < kMaxCPW; ++cc)
+#pragma unroll
+    for (int i = 0; i < kRPL; ++i) {
+      const int r = lane + 64 * i, c = wave * cpw + cc;
+      if (cc < cpw && r < h && c < l) out[(i64)r * ldo + c] = x[cc][i];
+    }
 }
 
 template <typename T>
@@ -273,12 +294,10 @@ void tsqr_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr
   ISVD_CHECK(l >= 1 && l <= 128, "TSQR supports 1 <= l <= 128 (got %lld)", l);
   ISVD_CHECK(m >= l, "TSQR needs m >= l (m = %lld, l = %lld)", m, l);
   ISVD_CHECK(batch >= 1 && batch <= 65535, "bad batch %lld", batch);
-  int LC = 1;
+  int LC = kWaves;
   while (LC < l) LC *= 2;
-  const int NRG = kNT / LC;
-  i64 hmax = (i64)NRG * kMaxR;  // rows one workgroup holds
-  if (hmax > kHMax) hmax = kHMax;
-  ISVD_CHECK(hmax >= 2 * l, "internal: panel capacity");
+  const i64 hmax = kHMax;  // rows one workgroup holds
+  ISVD_CHECK(hmax >= 2 * l && LC / kWaves <= kMaxCPW, "internal: panel capacity");
   PanelGeom geom{};
   geom.m = m;
   i64 nleaf = (m + hmax - 1) / hmax;
