@@ -92,7 +92,7 @@ def cpu_baseline(a_dev, omega_dev, k, p, N, it_full, m_sample):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--m", type=int, default=131072)
     ap.add_argument("--n", type=int, default=16384)
@@ -102,6 +102,8 @@ def main():
     ap.add_argument("--no-wy", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8192)
+    ap.add_argument("--sketch-only", action="store_true",
+                    help="run one sketch GEMM and one 1 GiB device copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this iSVD build runs on one GPU; --gpus must be 1"}))
@@ -115,6 +117,22 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     m, n, k, p, N = args.m, args.n, args.k, args.p, args.N
     l, Nl = k + p, (k + p) * N
+    if args.sketch_only:
+        # counter runs: the sketch GEMM once (HBM traffic per launch), then a plain copy of
+        # 2^30 bytes whose FETCH_SIZE / WRITE_SIZE calibrate the counters
+        a = torch.randn(m, n, device="cuda", dtype=torch.float64)
+        omega = torch.randn(n, Nl, device="cuda", dtype=torch.float64)
+        torch.cuda.synchronize()
+        prm = capi.create_param(m, n, k, p, N)
+        t0 = time.perf_counter()
+        pipeline.sketch(prm, a, omega=omega)
+        t1 = time.perf_counter()
+        src = torch.empty(2 ** 27, device="cuda", dtype=torch.float64).fill_(1.0)
+        dst = src.clone()
+        torch.cuda.synchronize()
+        print(json.dumps({"sketch_seconds": t1 - t0, "copy_bytes": 2 ** 30,
+                          "check": float(dst[0])}))
+        return
     r = min(2048, n)
     log(f"generating A {m}x{n} (FP64) and Omega {n}x{Nl} on the GPU")
     a, s_true, _ = make_problem(torch, m, n, r, seed=1234)
@@ -172,7 +190,8 @@ def main():
     achieved = flop / t_sketch
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
-                "kernel": "gemm_kernel<double,N,T,128,128,16> (sketch Y^T = Omega^T A^T)",
+                "kernel": "isvd::gemm_kernel<double,false,false,128,256,16,2,4> "
+                          "(sketch Y^T = Omega^T A^T, A row-major)",
                 "flop_per_launch": flop, "avg_launch_seconds": t_sketch}
     prof = os.path.join(ROOT, "profiles", "sketch_gemm_fp64_131072x16384_traffic.json")
     if os.path.exists(prof):

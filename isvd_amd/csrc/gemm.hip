@@ -8,6 +8,8 @@
 // the current tile's MFMAs (one barrier per K tile).
 #include "la.hpp"
 
+#include <cstdlib>
+
 namespace isvd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -52,6 +54,7 @@ struct GemmArgs {
   i64 kchunk;    // K elements per chunk (multiple of BK)
   T *W;          // split-K workspace: [batch][ksplit][m*n] dense col-major partials
   int group_m;   // raster group height (tiles) for L2 reuse
+  int upper;     // 1: only tiles touching the upper triangle (symmetric result, mirrored after)
 };
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN>
@@ -75,16 +78,43 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs<T> g) {
   const int wave = tid >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
 
-  // ---- tile coordinates (grouped raster) ----
+  // ---- tile coordinates ----
+  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Remap so
+  // that each XCD walks a contiguous range of the grouped raster: the tiles running together
+  // on one XCD then share operand panels through that XCD's L2.
   const i64 tiles_m = (g.m + BM - 1) / BM;
   const i64 tiles_n = (g.n + BN - 1) / BN;
-  const i64 pid = blockIdx.x;
-  const i64 in_group = (i64)g.group_m * tiles_n;
-  const i64 group_id = pid / in_group;
-  const i64 first_m = group_id * g.group_m;
-  const i64 gsz = (tiles_m - first_m) < g.group_m ? (tiles_m - first_m) : g.group_m;
-  const i64 tm = first_m + (pid % in_group) % gsz;
-  const i64 tn = (pid % in_group) / gsz;
+  // a symmetric product enumerates only the tiles touching the upper triangle
+  auto upper_count = [&](i64 tn) {
+    const i64 c = ((tn + 1) * BN + BM - 1) / BM;
+    return c < tiles_m ? c : tiles_m;
+  };
+  i64 total = tiles_m * tiles_n;
+  if (g.upper) {
+    total = 0;
+    for (i64 q = 0; q < tiles_n; ++q) total += upper_count(q);
+  }
+  i64 pid = blockIdx.x;
+  if (pid >= total) return;
+  {
+    constexpr i64 kXcd = 8;
+    const i64 xcd = pid % kXcd, local = pid / kXcd;
+    const i64 per = total / kXcd, rem = total % kXcd;
+    pid = (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+  }
+  i64 tm, tn;
+  if (g.upper) {  // column by column over the upper tiles
+    tn = 0;
+    while (pid >= upper_count(tn)) pid -= upper_count(tn++);
+    tm = pid;
+  } else {  // grouped raster: group_m tile rows walked column by column
+    const i64 in_group = (i64)g.group_m * tiles_n;
+    const i64 group_id = pid / in_group;
+    const i64 first_m = group_id * g.group_m;
+    const i64 gsz = (tiles_m - first_m) < g.group_m ? (tiles_m - first_m) : g.group_m;
+    tm = first_m + (pid % in_group) % gsz;
+    tn = (pid % in_group) / gsz;
+  }
   const i64 m0 = tm * BM, n0 = tn * BN;
 
   const int batch_id = blockIdx.y / g.ksplit;
@@ -222,6 +252,16 @@ __global__ void splitk_reduce_kernel(i64 m, i64 n, int ksplit, const T *__restri
   }
 }
 
+// C[i, j] := C[j, i] for i > j (n x n): completes a symmetric product computed on upper tiles
+template <typename T>
+__global__ void mirror_upper_kernel(i64 n, T *C, i64 ldc) {
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < n * n;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 i = e % n, j = e / n;
+    if (i > j) C[i + j * ldc] = C[j + i * ldc];
+  }
+}
+
 template <typename T>
 __global__ void scale_kernel(i64 m, i64 n, T beta, T *C, i64 ldc, i64 sC) {
   T *Cb = C + (i64)blockIdx.y * sC;
@@ -238,11 +278,20 @@ constexpr int kTargetBlocks = 1024;  // ~4 workgroups per CU on 256 CUs
 
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN>
 void launch_cfg(GemmArgs<T> g, i64 batch) {
-  const i64 tiles = ((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN);
+  const i64 tiles_m = (g.m + BM - 1) / BM, tiles_n = (g.n + BN - 1) / BN;
+  i64 tiles = tiles_m * tiles_n;
+  if (g.upper) {  // only tiles touching the upper triangle are launched
+    tiles = 0;
+    for (i64 q = 0; q < tiles_n; ++q) {
+      const i64 c = ((q + 1) * BN + BM - 1) / BM;
+      tiles += c < tiles_m ? c : tiles_m;
+    }
+  }
+  const i64 busy = tiles;
   // split-K when the output alone cannot fill the device
   int ksplit = 1;
-  if (tiles * batch < kTargetBlocks && g.k >= 8 * BK) {
-    i64 want = (kTargetBlocks + tiles * batch - 1) / (tiles * batch);
+  if (busy * batch < kTargetBlocks && g.k >= 8 * BK) {
+    i64 want = (kTargetBlocks + busy * batch - 1) / (busy * batch);
     i64 maxs = g.k / (4 * BK);
     if (want > maxs) want = maxs;
     if (want > 64) want = 64;
@@ -277,20 +326,38 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
   }
 }
 
+// ISVD_GEMM_TILE selects the large-tile configuration (for tuning sweeps):
+//   0: 128x256x16, 2x4 waves (default)   1: 128x128x16, 2x2 waves
+//   2: 256x128x16, 4x2 waves             3: 128x128x32, 2x2 waves
+int big_tile_cfg() {
+  static const int cfg = [] {
+    const char *e = std::getenv("ISVD_GEMM_TILE");
+    return e ? std::atoi(e) : 0;
+  }();
+  return cfg;
+}
+
 template <typename T, bool TA, bool TB>
 void launch(GemmArgs<T> g, i64 batch) {
   const bool big = (g.m >= 128 && g.n >= 128);
-  if (big)
-    launch_cfg<T, TA, TB, 128, 128, 16, 2, 2>(g, batch);
-  else
+  if (!big) {
     launch_cfg<T, TA, TB, 64, 64, 16, 2, 2>(g, batch);
+    return;
+  }
+  switch (big_tile_cfg()) {
+    case 1: launch_cfg<T, TA, TB, 128, 128, 16, 2, 2>(g, batch); break;
+    case 2: launch_cfg<T, TA, TB, 256, 128, 16, 4, 2>(g, batch); break;
+    case 3: launch_cfg<T, TA, TB, 128, 128, 32, 2, 2>(g, batch); break;
+    default: launch_cfg<T, TA, TB, 128, 256, 16, 2, 4>(g, batch); break;
+  }
 }
 
 }  // namespace
 
 template <typename T>
-void gemm(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda,
-          const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch, i64 sA, i64 sB, i64 sC) {
+static void gemm_impl(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A,
+                      i64 lda, const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch, i64 sA,
+                      i64 sB, i64 sC, int upper) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
   const bool ta = (transa == 'T' || transa == 't' || transa == 'C' || transa == 'c');
   const bool tb = (transb == 'T' || transb == 't' || transb == 'C' || transb == 'c');
@@ -308,12 +375,35 @@ void gemm(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i6
   GemmArgs<T> g{};
   g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
   g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
-  g.sA = sA; g.sB = sB; g.sC = sC; g.W = nullptr;
+  g.sA = sA; g.sB = sB; g.sC = sC; g.W = nullptr; g.upper = upper;
   if (!ta && !tb) launch<T, false, false>(g, batch);
   else if (!ta && tb) launch<T, false, true>(g, batch);
   else if (ta && !tb) launch<T, true, false>(g, batch);
   else launch<T, true, true>(g, batch);
 }
+
+template <typename T>
+void gemm(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda,
+          const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch, i64 sA, i64 sB, i64 sC) {
+  gemm_impl<T>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB, sC,
+               0);
+}
+
+template <typename T>
+void syrk_full(char trans, i64 n, i64 k, T alpha, const T *A, i64 lda, T *C, i64 ldc) {
+  // C = alpha op(A) op(A)^T with op(A) = A (trans 'N', n x k) or A^T (trans 'T', A is k x n);
+  // only tiles that touch the upper triangle are computed, then mirrored
+  const char ta = (trans == 'T' || trans == 't') ? 'T' : 'N';
+  const char tb = (ta == 'T') ? 'N' : 'T';
+  gemm_impl<T>(ta, tb, n, n, k, alpha, A, lda, A, lda, T(0), C, ldc, 1, 0, 0, 0, 1);
+  int blocks = (int)((n * n + 255) / 256);
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(mirror_upper_kernel<T>, dim3(blocks), dim3(256), 0, ctx().stream, n, C, ldc);
+  LAUNCH_CHECK();
+}
+
+template void syrk_full<double>(char, i64, i64, double, const double *, i64, double *, i64);
+template void syrk_full<float>(char, i64, i64, float, const float *, i64, float *, i64);
 
 template void gemm<double>(char, char, i64, i64, i64, double, const double *, i64,
                            const double *, i64, double, double *, i64, i64, i64, i64, i64);

@@ -17,6 +17,11 @@ void gemm(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i6
           const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch = 1, i64 sA = 0, i64 sB = 0,
           i64 sC = 0);
 
+// C := alpha op(A) op(A)^T (n x n, full storage): op(A) = A (trans 'N', A n x k) or A^T
+// (trans 'T', A k x n).  Computes the upper-triangle tiles and mirrors them.
+template <typename T>
+void syrk_full(char trans, i64 n, i64 k, T alpha, const T *A, i64 lda, T *C, i64 ldc);
+
 // B := alpha * op(A)   (m x n result; mkl_?omatcopy semantics, 'N' or 'T').
 template <typename T>
 void omatcopy(char trans, i64 m, i64 n, T alpha, const T *A, i64 lda, T *B, i64 ldb,

@@ -32,6 +32,8 @@ template <> struct SM<float> {
   static __device__ __forceinline__ int row(int lane, int g) { return 4 * (lane >> 4) + g; }
 };
 
+constexpr int kSmallMaxK = 128;
+
 template <typename T>
 struct SmallProblem {
   const T *A;
@@ -52,13 +54,20 @@ __global__ __launch_bounds__(64) void small_gemm_kernel(int m, int n, int k, Sma
   const int lane = threadIdx.x, lr = lane & 15, lk = lane >> 4;
   const int row = tm * 16 + lr, col = tn * 16 + lr;
   typename SM<T>::acc acc = {0, 0, 0, 0};
-  for (int k0 = 0; k0 < k; k0 += 4) {
-    const int kk = k0 + lk;
+  // all fragments are loaded before the MFMA chain so the L2 round trips overlap (k <= 128)
+  T fa[kSmallMaxK / 4], fb[kSmallMaxK / 4];
+#pragma unroll
+  for (int s = 0; s < kSmallMaxK / 4; ++s) {
+    const int kk = 4 * s + lk;
     T a = 0, b = 0;
     if (row < m && kk < k) a = p.ta ? p.A[kk + (i64)row * p.lda] : p.A[row + (i64)kk * p.lda];
     if (col < n && kk < k) b = p.tb ? p.B[col + (i64)kk * p.ldb] : p.B[kk + (i64)col * p.ldb];
-    acc = SM<T>::mfma(a, b, acc);
+    fa[s] = a;
+    fb[s] = b;
   }
+#pragma unroll
+  for (int s = 0; s < kSmallMaxK / 4; ++s)
+    if (4 * s < k) acc = SM<T>::mfma(fa[s], fb[s], acc);
 #pragma unroll
   for (int g = 0; g < 4; ++g) {
     const int r = tm * 16 + SM<T>::row(lane, g), c = tn * 16 + lr;
@@ -72,6 +81,7 @@ __global__ __launch_bounds__(64) void small_gemm_kernel(int m, int n, int k, Sma
 
 template <typename T>
 void small_gemm2(int m, int n, int k, const SmallProblem<T> &p0, const SmallProblem<T> *p1) {
+  ISVD_CHECK(k >= 0 && k <= kSmallMaxK, "small_gemm: k = %d > %d", k, kSmallMaxK);
   const int tiles = ((m + 15) / 16) * ((n + 15) / 16);
   hipLaunchKernelGGL(small_gemm_kernel<T>, dim3(tiles, p1 ? 2 : 1), dim3(64), 0, ctx().stream, m,
                      n, k, p0, p1 ? *p1 : p0);

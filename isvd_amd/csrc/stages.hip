@@ -256,7 +256,7 @@ void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldy
   DBuf<T> dc(l * l), z(l * l), w(l), zv(l * l), zs(l * l), zi(l * l), c(l * l), cinv(l * l),
       err(1);
   // M := Qs^T Qs
-  gemm<T>('N', 'T', Nl, Nl, m, T(1), yst, ldyst, yst, ldyst, T(0), M.get(), Nl);
+  syrk_full<T>('N', Nl, m, T(1), yst, ldyst, M.get(), Nl);
   // X_c := [I; 0] (Qc := Q_0, :156);  B_c := Qs^T Q_0 = M[:, 0:l]  (:159-160)
   fill<T>(Nl, l, T(0), xc.get(), Nl, T(1));
   omatcopy<T>('N', Nl, l, T(1), M.get(), Nl, bc.get(), Nl);
@@ -340,7 +340,7 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
       bgc(Nl * l), gq(Nl * l), d1(Nl * l), d2(Nl * l), md1(Nl * l), md2(Nl * l);
   DBuf<T> dc(l * l), dgc(l * l), zm(l * l), si(l * l), fcb(l * l), fgcb(l * l), scal(4);
   DBuf<int> info(1);
-  gemm<T>('N', 'T', Nl, Nl, m, T(1), yst, ldyst, yst, ldyst, T(0), M.get(), Nl);
+  syrk_full<T>('N', Nl, m, T(1), yst, ldyst, M.get(), Nl);
   // init (:201-235)
   fill<T>(Nl, l, T(0), xq.get(), Nl, T(1));
   omatcopy<T>('N', Nl, l, T(1), M.get(), Nl, bc.get(), Nl);

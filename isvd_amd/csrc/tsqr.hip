@@ -6,8 +6,8 @@
 //  src/lib/libisvd/core/stage/@x@_postprocess_tall_skinny_qr.c:184-246).  On one GPU the same
 // tree runs over row panels of each sketch:
 //   level 0   : every row panel (<= 256 rows) is factored by one workgroup with the panel held
-//               in registers (each wave owns whole columns, each lane four rows), Householder
-//               vectors written back in place, R written to the next level's buffer;
+//               in registers (each wave owns whole columns), Householder vectors written back
+//               in place, R written to the next level's buffer;
 //   level k>0 : two consecutive R factors form a stacked 2l x l panel that is factored the
 //               same way, until one R per sketch remains;
 //   formation : top-down, each node applies its reflectors to [C; 0] (C = I at the root) and
@@ -20,14 +20,17 @@ namespace isvd {
 namespace {
 
 // Panel layout: 8 waves; wave w owns columns [w*cpw, (w+1)*cpw) of the panel (cpw = LC / 8,
-// LC = l rounded up to a power of two >= 8), lane owns rows lane + 64 i (i < 4).  Column norms
-// and the products v^T A[:, c] are wave reductions (shuffles); the Householder vector of
-// each step is broadcast through a double-buffered LDS vector, one barrier per step.
+// LC = l rounded up to a power of two >= 8).  Inside a wave, lane = 16 * cs + rg: the lane
+// holds rows rg + 16 i (i < 16) of the columns w*cpw + cs + 4 t (t < 4).  Column norms and
+// the products v^T A[:, c] are reductions over the 16 lanes that share cs (four shuffle
+// levels, issued level by level for all of the lane's columns at once); the Householder
+// vector of each step is broadcast through a double-buffered LDS vector, one barrier per step.
 constexpr int kWaves = 8;
 constexpr int kNT = 64 * kWaves;  // threads per panel
-constexpr int kRPL = 4;           // rows per lane
+constexpr int kRPL = 16;          // rows per lane
+constexpr int kCPL = 4;           // columns per lane
 constexpr int kMaxCPW = 16;       // columns per wave (l <= 128)
-constexpr int kHMax = 64 * kRPL;  // rows per panel
+constexpr int kHMax = 16 * kRPL;  // rows per panel
 
 struct PanelGeom {
   i64 m;        // rows of the level-0 matrix
@@ -74,10 +77,19 @@ __device__ __forceinline__ void panel_of(const LevelArgs &a, int b, int j, T *&b
   }
 }
 
+// sum over the 16 lanes of a row group (xor 8, 4, 2, 1), for kCPL values at once
 template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
+__device__ __forceinline__ void group16_sum(T (&v)[kCPL]) {
 #pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  for (int off = 8; off > 0; off >>= 1)
+#pragma unroll
+    for (int t = 0; t < kCPL; ++t) v[t] += __shfl_xor(v[t], off, 16);
+}
+
+template <typename T>
+__device__ __forceinline__ T group16_sum1(T v) {
+#pragma unroll
+  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
   return v;
 }
 
@@ -88,45 +100,55 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
   const int j_node = blockIdx.x, b = blockIdx.y;
   const int l = a.l, cpw = a.LC / kWaves;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int cs = lane >> 4, rg = lane & 15;
   T *base;
   i64 ld;
   int h;
   panel_of<T>(a, b, j_node, base, ld, h);
   for (int e = tid; e < 2 * kHMax; e += kNT) (&vsh[0][0])[e] = T(0);
 
-  T x[kMaxCPW][kRPL];
+  // column of slot t for this lane, or -1
+  auto colof = [&](int t) {
+    const int lc = cs + 4 * t;
+    const int c = wave * cpw + lc;
+    return (lc < cpw && c < l) ? c : -1;
+  };
+  T x[kCPL][kRPL];
 #pragma unroll
-  for (int cc = 0; cc < kMaxCPW; ++cc)
+  for (int t = 0; t < kCPL; ++t) {
+    const int c = colof(t);
 #pragma unroll
     for (int i = 0; i < kRPL; ++i) {
-      const int r = lane + 64 * i, c = wave * cpw + cc;
-      x[cc][i] = (cc < cpw && r < h && c < l) ? base[(i64)r * ld + c] : T(0);
+      const int r = rg + 16 * i;
+      x[t][i] = (c >= 0 && r < h) ? base[(i64)r * ld + c] : T(0);
     }
+  }
   const int kmax = h < l ? h : l;
   T *tau_out = reinterpret_cast<T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
   __syncthreads();
 
   for (int j = 0; j < kmax; ++j) {
     const int buf = j & 1;
-    if (wave == j / cpw) {
-      const int jj = j % cpw;
+    const int lj = j - wave * cpw;  // column j relative to this wave
+    if (lj >= 0 && lj < cpw && cs == (lj & 3)) {
+      const int tj = lj >> 2;
       T col[kRPL];
 #pragma unroll
       for (int i = 0; i < kRPL; ++i) col[i] = T(0);
 #pragma unroll
-      for (int cc = 0; cc < kMaxCPW; ++cc)
-        if (cc == jj)
+      for (int t = 0; t < kCPL; ++t)
+        if (t == tj)
 #pragma unroll
-          for (int i = 0; i < kRPL; ++i) col[i] = x[cc][i];
+          for (int i = 0; i < kRPL; ++i) col[i] = x[t][i];
       T part = 0, av = 0;
 #pragma unroll
       for (int i = 0; i < kRPL; ++i) {
-        const int r = lane + 64 * i;
+        const int r = rg + 16 * i;
         if (r > j) part += col[i] * col[i];  // rows >= h hold zeros
         if (r == j) av = col[i];
       }
-      const T sigma2 = wave_sum(part);
-      const T alpha = wave_sum(av);  // one lane contributes
+      const T sigma2 = group16_sum1(part);
+      const T alpha = group16_sum1(av);  // one lane contributes
       T beta, tau, scale;
       if (sigma2 == T(0)) {
         beta = alpha; tau = T(0); scale = T(0);
@@ -138,17 +160,17 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
       }
 #pragma unroll
       for (int i = 0; i < kRPL; ++i) {
-        const int r = lane + 64 * i;
+        const int r = rg + 16 * i;
         if (r > j) col[i] *= scale;
         else if (r == j) col[i] = beta;
-        if (r < kHMax) vsh[buf][r] = (r > j && r < h) ? col[i] : (r == j ? T(1) : T(0));
+        vsh[buf][r] = (r > j && r < h) ? col[i] : (r == j ? T(1) : T(0));
       }
 #pragma unroll
-      for (int cc = 0; cc < kMaxCPW; ++cc)
-        if (cc == jj)
+      for (int t = 0; t < kCPL; ++t)
+        if (t == tj)
 #pragma unroll
-          for (int i = 0; i < kRPL; ++i) x[cc][i] = col[i];
-      if (lane == 0) {
+          for (int i = 0; i < kRPL; ++i) x[t][i] = col[i];
+      if (rg == 0) {
         tau_out[j] = tau;
         tau_sh[buf] = tau;
       }
@@ -158,17 +180,21 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
     if (tau != T(0)) {
       T v[kRPL];
 #pragma unroll
-      for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][lane + 64 * i];
+      for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][rg + 16 * i];
+      T part[kCPL];
 #pragma unroll
-      for (int cc = 0; cc < kMaxCPW; ++cc) {
-        const int c = wave * cpw + cc;
-        if (cc < cpw && c > j && c < l) {
-          T part = 0;
+      for (int t = 0; t < kCPL; ++t) {
+        part[t] = T(0);
 #pragma unroll
-          for (int i = 0; i < kRPL; ++i) part += v[i] * x[cc][i];
-          const T w = wave_sum(part) * tau;
+        for (int i = 0; i < kRPL; ++i) part[t] += v[i] * x[t][i];
+      }
+      group16_sum(part);
 #pragma unroll
-          for (int i = 0; i < kRPL; ++i) x[cc][i] -= v[i] * w;
+      for (int t = 0; t < kCPL; ++t) {
+        if (colof(t) > j) {
+          const T w = part[t] * tau;
+#pragma unroll
+          for (int i = 0; i < kRPL; ++i) x[t][i] -= v[i] * w;
         }
       }
     }
@@ -176,15 +202,16 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
   // V (below the diagonal) and R (on and above) back in place; R to the next level
   T *R = reinterpret_cast<T *>(a.Rout) + ((i64)b * a.nnode + j_node) * l * l;
 #pragma unroll
-  for (int cc = 0; cc < kMaxCPW; ++cc)
+  for (int t = 0; t < kCPL; ++t) {
+    const int c = colof(t);
+    if (c < 0) continue;
 #pragma unroll
     for (int i = 0; i < kRPL; ++i) {
-      const int r = lane + 64 * i, c = wave * cpw + cc;
-      if (cc < cpw && c < l) {
-        if (r < h) base[(i64)r * ld + c] = x[cc][i];
-        if (r < l) R[r * l + c] = (r <= c && r < kmax) ? x[cc][i] : T(0);
-      }
+      const int r = rg + 16 * i;
+      if (r < h) base[(i64)r * ld + c] = x[t][i];
+      if (r < l) R[r * l + c] = (r <= c && r < kmax) ? x[t][i] : T(0);
     }
+  }
 }
 
 // Explicit Q: X := H_0 H_1 ... H_{k-1} [C; 0], reflectors applied last-to-first.  The vector of
@@ -195,6 +222,7 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
   const int j_node = blockIdx.x, b = blockIdx.y;
   const int l = a.l, cpw = a.LC / kWaves;
   const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int cs = lane >> 4, rg = lane & 15;
   T *base;
   i64 ld;
   int h;
@@ -202,52 +230,62 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
   const int kmax = h < l ? h : l;
   const T *C = reinterpret_cast<const T *>(a.Cin) + ((i64)b * a.nnode + j_node) * l * l;
   const T *tau_in = reinterpret_cast<const T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
-  T x[kMaxCPW][kRPL];
+  auto colof = [&](int t) {
+    const int lc = cs + 4 * t;
+    const int c = wave * cpw + lc;
+    return (lc < cpw && c < l) ? c : -1;
+  };
+  T x[kCPL][kRPL];
 #pragma unroll
-  for (int cc = 0; cc < kMaxCPW; ++cc)
+  for (int t = 0; t < kCPL; ++t) {
+    const int c = colof(t);
 #pragma unroll
     for (int i = 0; i < kRPL; ++i) {
-      const int r = lane + 64 * i, c = wave * cpw + cc;
-      x[cc][i] = (cc < cpw && r < kmax && c < l) ? C[r * l + c] : T(0);
+      const int r = rg + 16 * i;
+      x[t][i] = (c >= 0 && r < kmax) ? C[r * l + c] : T(0);
     }
-  // v_j[r] = 0 (r < j), 1 (r == j), V[r, j] (j < r < h), 0 (r >= h)
+  }
+  // v_j[r] = 0 (r < j), 1 (r == j), V[r, j] (j < r < h), 0 (r >= h); 64 lanes x 4 entries
   auto vload = [&](int j, int r) -> T {
     return (r < j || r >= h) ? T(0) : (r == j ? T(1) : base[(i64)r * ld + j]);
   };
   if (kmax > 0 && wave == 0) {
 #pragma unroll
-    for (int i = 0; i < kRPL; ++i) vsh[(kmax - 1) & 1][lane + 64 * i] = vload(kmax - 1, lane + 64 * i);
+    for (int q = 0; q < kHMax / 64; ++q)
+      vsh[(kmax - 1) & 1][lane + 64 * q] = vload(kmax - 1, lane + 64 * q);
   }
   __syncthreads();
   for (int j = kmax - 1; j >= 0; --j) {
     const int buf = j & 1;
     const bool loader = (j > 0) && (wave == (j - 1) % kWaves);
-    T vnext[kRPL];
+    T vnext[kHMax / 64];
     if (loader) {
 #pragma unroll
-      for (int i = 0; i < kRPL; ++i) vnext[i] = vload(j - 1, lane + 64 * i);
+      for (int q = 0; q < kHMax / 64; ++q) vnext[q] = vload(j - 1, lane + 64 * q);
     }
     const T tau = tau_in[j];
     if (tau != T(0)) {
       T v[kRPL];
 #pragma unroll
-      for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][lane + 64 * i];
+      for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][rg + 16 * i];
+      T part[kCPL];
 #pragma unroll
-      for (int cc = 0; cc < kMaxCPW; ++cc) {
-        const int c = wave * cpw + cc;
-        if (cc < cpw && c < l) {
-          T part = 0;
+      for (int t = 0; t < kCPL; ++t) {
+        part[t] = T(0);
 #pragma unroll
-          for (int i = 0; i < kRPL; ++i) part += v[i] * x[cc][i];
-          const T w = wave_sum(part) * tau;
+        for (int i = 0; i < kRPL; ++i) part[t] += v[i] * x[t][i];
+      }
+      group16_sum(part);
 #pragma unroll
-          for (int i = 0; i < kRPL; ++i) x[cc][i] -= v[i] * w;
-        }
+      for (int t = 0; t < kCPL; ++t) {
+        const T w = part[t] * tau;
+#pragma unroll
+        for (int i = 0; i < kRPL; ++i) x[t][i] -= v[i] * w;
       }
     }
     if (loader) {
 #pragma unroll
-      for (int i = 0; i < kRPL; ++i) vsh[buf ^ 1][lane + 64 * i] = vnext[i];
+      for (int q = 0; q < kHMax / 64; ++q) vsh[buf ^ 1][lane + 64 * q] = vnext[q];
     }
     __syncthreads();
   }
@@ -260,12 +298,15 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
     ldo = l;
   }
 #pragma unroll
-  for (int cc = 0; cc < kMaxCPW; ++cc)
+  for (int t = 0; t < kCPL; ++t) {
+    const int c = colof(t);
+    if (c < 0) continue;
 #pragma unroll
     for (int i = 0; i < kRPL; ++i) {
-      const int r = lane + 64 * i, c = wave * cpw + cc;
-      if (cc < cpw && r < h && c < l) out[(i64)r * ldo + c] = x[cc][i];
+      const int r = rg + 16 * i;
+      if (r < h) out[(i64)r * ldo + c] = x[t][i];
     }
+  }
 }
 
 template <typename T>
