@@ -25,8 +25,9 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_problem(torch, m, n, r, seed, noise=1e-6, dtype=None):
-    """A = U diag(1/i) V^T + noise, generated on the GPU (torch; not part of the timed path)."""
+def make_problem(torch, m, n, r, seed, noise=1e-6, dtype=None, spectrum=None):
+    """A = U diag(s) V^T + noise, s_i = 1/i by default, generated on the GPU (torch; not part
+    of the timed path)."""
     dtype = dtype or torch.float64
     g = torch.Generator(device="cuda").manual_seed(seed)
 
@@ -41,7 +42,10 @@ def make_problem(torch, m, n, r, seed, noise=1e-6, dtype=None):
 
     u = orth(m)
     v = orth(n)
-    s = 1.0 / torch.arange(1, r + 1, device="cuda", dtype=torch.float64)
+    if spectrum is None:
+        s = 1.0 / torch.arange(1, r + 1, device="cuda", dtype=torch.float64)
+    else:
+        s = torch.as_tensor(spectrum, device="cuda", dtype=torch.float64)
     us = u * s[None, :]
     a = torch.empty(m, n, device="cuda", dtype=dtype)
     blk = 16384
@@ -89,6 +93,60 @@ def cpu_baseline(a_dev, omega_dev, k, p, N, it_full, m_sample):
             "stage_seconds_sample": [t1 - t0, t2 - t1, t3 - t2, t4 - t3]}
 
 
+def principal_angle_max(torch, a, b):
+    """Largest principal angle (radians) between span(a) and span(b) (orthonormal columns)."""
+    p = b - a @ (a.T @ b)
+    return float(torch.arcsin(torch.clamp(torch.linalg.svdvals(p)[0], 0.0, 1.0)))
+
+
+def mixed_case(torch, capi, pipeline, m, n, k, p, N, r, decay, steps, warmup):
+    """BF16 A (spectrum 0.95^i or i^-0.5, + 1e-6 noise) through the mixed-precision driver,
+    timed, and compared with the FP64 driver on the identical BF16 values of A and Omega."""
+    import numpy as np
+    i = np.arange(r, dtype=np.float64)
+    spectrum = 0.95 ** i if decay == "fast" else (i + 1.0) ** -0.5
+    a64, s_true, _ = make_problem(torch, m, n, r, seed=4321, spectrum=spectrum)
+    a_b = a64.to(torch.bfloat16)
+    a64.copy_(a_b)  # FP64 copy of exactly the BF16 values
+    gen = torch.Generator(device="cuda").manual_seed(77)
+    om_b = torch.randn(n, (k + p) * N, generator=gen, device="cuda",
+                       dtype=torch.float32).to(torch.bfloat16)
+    prm = capi.create_param(m, n, k, p, N)
+    algs = ("GP", "TS", "KN", "TS")
+    for _ in range(warmup):
+        rb = pipeline.isvd(prm, a_b, omega=om_b, algs=algs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stages = []
+    for _ in range(steps):
+        rb = pipeline.isvd(prm, a_b, omega=om_b, algs=algs)
+        stages.append(rb["time"])
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / steps
+    log(f"mixed ({decay} decay): {sec:.4f} s, stages {np.mean(stages, axis=0).round(4).tolist()}, "
+        f"KN iter {rb['iter']}")
+    om64 = om_b.double()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    rd = pipeline.isvd(prm, a64, omega=om64, algs=algs)
+    torch.cuda.synchronize()
+    sec64 = time.perf_counter() - t0
+    sb, sd = rb["s"][:k].double(), rd["s"][:k]
+    out = {"decay": "s_i=0.95^i" if decay == "fast" else "s_i=i^-0.5", "seconds": sec,
+           "stage_seconds": np.mean(stages, axis=0).tolist(), "kn_iterations": rb["iter"],
+           "fp64_seconds": sec64, "fp64_stage_seconds": rd["time"], "speedup_vs_fp64": sec64 / sec,
+           "fp64_kn_iterations": rd["iter"],
+           "sigma_max_rel_err_vs_fp64": float(((sb - sd).abs() / sd).max()),
+           "u_angle_max_vs_fp64": principal_angle_max(torch, rb["u"][:, :k].double(),
+                                                      rd["u"][:, :k]),
+           "v_angle_max_vs_fp64": principal_angle_max(torch, rb["v"][:, :k].double(),
+                                                      rd["v"][:, :k]),
+           "sigma_max_rel_err_vs_planted": float(((sb - s_true[:k]).abs() / s_true[:k]).max())}
+    del a64, a_b, rb, rd
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -100,6 +158,7 @@ def main():
     ap.add_argument("--p", type=int, default=28)
     ap.add_argument("--N", type=int, default=32)
     ap.add_argument("--no-wy", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-rows", type=int, default=8192)
     ap.add_argument("--sketch-only", action="store_true",
@@ -198,6 +257,19 @@ def main():
         with open(prof) as f:
             roofline["traffic"] = json.load(f).get("hbm_bytes_per_launch")
 
+    mixed = None
+    if not args.no_mixed:
+        del a
+        torch.cuda.empty_cache()
+        mixed = {"workload": f"A {m}x{n} stored in BF16 (U diag(s) V^T + 1e-6 noise, rank {r}), "
+                             f"k={k} p={p} N={N}, GP/TS/KN/TS; BF16 MFMA sketch and projection, "
+                             f"split-BF16 x3 Gram, FP64 l x l solves; errors vs the FP64 driver "
+                             f"on the same BF16 values",
+                 "cases": [mixed_case(torch, capi, pipeline, m, n, k, p, N, r, decay,
+                                      args.steps, args.warmup) for decay in ("fast", "slow")]}
+        mixed["speedup_vs_fp64_same_input"] = min(c["speedup_vs_fp64"] for c in mixed["cases"])
+        a, _, _ = make_problem(torch, m, n, r, seed=1234)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle on a row sample)")
@@ -214,7 +286,7 @@ def main():
         "stage_seconds": stage_avg, "kn_iterations": res["iter"],
         "accuracy": {"sigma_max_rel_err_vs_planted": sigma_rel_err, "UtU_minus_I": orth_u,
                      "VtV_minus_I": orth_v, "residual_AV_minus_US_rel": resid},
-        "wen_yin": wy, "roofline": roofline, "cpu_baseline": cpu,
+        "wen_yin": wy, "mixed": mixed, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

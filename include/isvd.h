@@ -213,6 +213,72 @@ ISVD_DECLARE_DRIVER_OMEGA(d, isvd_d_val_t)
 ISVD_DECLARE_STAGES(s, isvd_s_val_t)
 ISVD_DECLARE_STAGES(d, isvd_d_val_t)
 
+/* =========================================================================================
+ * Mixed precision, the "b" family (no counterpart in the reference; same argument order and
+ * meaning as the d family above).  A and Omega are stored as BF16 (isvd_b_val_t holds the
+ * bit pattern: the upper half of the FP32 value); every other array is FP32.
+ *   sketch       Y = A Omega on BF16 matrix cores with FP32 accumulation;
+ *   alg_o "GR"   Gramian orthogonalization applied twice in FP32 (FP64 l x l eigensolves);
+ *   alg_o "TS"   FP32 Householder TSQR;
+ *   alg_i        M = Qs^T Qs from split-BF16 x3 compensated FP32 products (hi*hi + hi*lo +
+ *                lo*hi), the KN / WY iterations in FP64 on l x l and N*l x l matrices;
+ *   alg_p        Z^T = Qbar^T A with Qbar split into three BF16 terms (exact against the BF16
+ *                A, FP32 accumulation), then the l x l factorizations, U and V in FP64.
+ * The seeded forms draw Omega from the FP32 Philox stream of isvd_sIsvd, rounded to BF16.
+ * ========================================================================================= */
+typedef uint16_t isvd_b_val_t;
+void isvd_bIsvd(const isvd_Param param, const char *alg_s, const char *alg_o, const char *alg_i,
+                const char *alg_p, const float *argv[4], const isvd_int_t argc[4],
+                float *retv[4], const isvd_int_t retc[4], double time[4], FILE *stream,
+                const char dista, const char ordera, const isvd_b_val_t *a,
+                const isvd_int_t lda, float *s, float *ut, const isvd_int_t ldut, float *vt,
+                const isvd_int_t ldvt, const isvd_int_t seed, const mpi_int_t ut_root,
+                const mpi_int_t vt_root);
+void isvd_bIsvdOmega(const isvd_Param param, const char *alg_s, const char *alg_o,
+                     const char *alg_i, const char *alg_p, const float *argv[4],
+                     const isvd_int_t argc[4], float *retv[4], const isvd_int_t retc[4],
+                     double time[4], FILE *stream, const char dista, const char ordera,
+                     const isvd_b_val_t *a, const isvd_int_t lda, const isvd_b_val_t *omegat,
+                     const isvd_int_t ldomegat, float *s, float *ut, const isvd_int_t ldut,
+                     float *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,
+                     const mpi_int_t vt_root);
+void isvd_bSketchGaussianProjection(const isvd_Param param, const float *argv,
+                                    const isvd_int_t argc, float *retv, const isvd_int_t retc,
+                                    const char dista, const char ordera, const isvd_b_val_t *a,
+                                    const isvd_int_t lda, float *yst, const isvd_int_t ldyst,
+                                    const isvd_int_t seed, const mpi_int_t mpi_root);
+void isvd_bSketchGaussianProjectionOmega(const isvd_Param param, const char dista,
+                                         const char ordera, const isvd_b_val_t *a,
+                                         const isvd_int_t lda, const isvd_b_val_t *omegat,
+                                         const isvd_int_t ldomegat, float *yst,
+                                         const isvd_int_t ldyst);
+void isvd_bOrthogonalizeTallSkinnyQr(const isvd_Param param, const float *argv,
+                                     const isvd_int_t argc, float *retv, const isvd_int_t retc,
+                                     float *yst, const isvd_int_t ldyst);
+void isvd_bOrthogonalizeGramian(const isvd_Param param, const float *argv, const isvd_int_t argc,
+                                float *retv, const isvd_int_t retc, float *yst,
+                                const isvd_int_t ldyst);
+void isvd_bIntegrateKolmogorovNagumo(const isvd_Param param, const float *argv,
+                                     const isvd_int_t argc, float *retv, const isvd_int_t retc,
+                                     const float *yst, const isvd_int_t ldyst, float *qt,
+                                     const isvd_int_t ldqt);
+void isvd_bIntegrateWenYin(const isvd_Param param, const float *argv, const isvd_int_t argc,
+                           float *retv, const isvd_int_t retc, const float *yst,
+                           const isvd_int_t ldyst, float *qt, const isvd_int_t ldqt);
+void isvd_bPostprocessTallSkinnyQr(const isvd_Param param, const float *argv,
+                                   const isvd_int_t argc, float *retv, const isvd_int_t retc,
+                                   const char dista, const char ordera, const isvd_b_val_t *a,
+                                   const isvd_int_t lda, const float *qt, const isvd_int_t ldqt,
+                                   float *s, float *ut, const isvd_int_t ldut, float *vt,
+                                   const isvd_int_t ldvt, const mpi_int_t ut_root,
+                                   const mpi_int_t vt_root);
+void isvd_bPostprocessGramian(const isvd_Param param, const float *argv, const isvd_int_t argc,
+                              float *retv, const isvd_int_t retc, const char dista,
+                              const char ordera, const isvd_b_val_t *a, const isvd_int_t lda,
+                              const float *qt, const isvd_int_t ldqt, float *s, float *ut,
+                              const isvd_int_t ldut, float *vt, const isvd_int_t ldvt,
+                              const mpi_int_t ut_root, const mpi_int_t vt_root);
+
 #if defined(__cplusplus)
 }
 #endif

@@ -5,6 +5,9 @@ src/include/c/isvd/core/@x@_stage.h:20-77, @x@_driver.h:21-29).  Matrix argument
 addresses: a numpy array (host memory, staged through HBM by the library) or a device
 address (e.g. ``torch.Tensor.data_ptr()`` of a CUDA tensor, used in place).
 
+The "b" family (mixed precision) takes A and Omega as BF16 bit patterns (numpy uint16 or
+torch.bfloat16) and every other array as FP32.
+
 Storage conventions (all column-major, as in the reference):
   a      : A, m x n with lda >= m (ordera 'C'), or the row-major m x n A with lda >= n ('R')
   omegat : Omega^T, Nl x n, ldomegat >= Nl          (= row-major Omega, n x Nl)
@@ -62,10 +65,11 @@ def load(path=None):
     _sig(lib.isvd_Init_gpu, [VP, VP, ctypes.c_int])
     _sig(lib.isvd_Finalize_gpu, [])
     _sig(lib.isvd_PrintEnvironment_gpu, [ctypes.c_int])
-    for x in "sd":
+    for x in "sdb":
         sketch = [Param, VP, I, VP, I, CH, CH, VP, I, VP, I, I, mpi_int_t]
         _sig(getattr(lib, f"isvd_{x}SketchGaussianProjection"), sketch)
-        _sig(getattr(lib, f"isvd_{x}SketchGaussianProjection_gpu"), sketch)
+        if x != "b":
+            _sig(getattr(lib, f"isvd_{x}SketchGaussianProjection_gpu"), sketch)
         _sig(getattr(lib, f"isvd_{x}SketchGaussianProjectionOmega"),
              [Param, CH, CH, VP, I, VP, I, VP, I])
         orth = [Param, VP, I, VP, I, VP, I]
@@ -75,8 +79,10 @@ def load(path=None):
         _sig(getattr(lib, f"isvd_{x}IntegrateKolmogorovNagumo"), integ)
         _sig(getattr(lib, f"isvd_{x}IntegrateWenYin"), integ)
         post = [Param, VP, I, VP, I, CH, CH, VP, I, VP, I, VP, VP, I, VP, I, mpi_int_t, mpi_int_t]
-        for nm in ("PostprocessTallSkinnyQr", "PostprocessGramian",
-                   "PostprocessTallSkinnyQr_gpu", "PostprocessGramian_gpu"):
+        names = ("PostprocessTallSkinnyQr", "PostprocessGramian")
+        if x != "b":
+            names += ("PostprocessTallSkinnyQr_gpu", "PostprocessGramian_gpu")
+        for nm in names:
             _sig(getattr(lib, f"isvd_{x}{nm}"), post)
         drv = [Param, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                VP, VP, VP, VP, VP, VP, CH, CH, VP, I]
@@ -114,7 +120,21 @@ def _chr(c):
 
 
 def prefix(dtype):
-    return "d" if np.dtype(dtype) == np.float64 else "s"
+    """Precision family of the entry points: 'd' (FP64), 's' (FP32) or 'b' (BF16 A and Omega,
+    given as uint16 bit patterns or torch.bfloat16; FP32 everywhere else)."""
+    name = str(dtype)
+    if name in ("torch.bfloat16", "bfloat16") or name.endswith("uint16"):
+        return "b"
+    if name in ("torch.float64", "float64") or name.endswith("float64'>"):
+        return "d"
+    try:
+        if np.dtype(dtype) == np.float64:
+            return "d"
+        if np.dtype(dtype) == np.uint16:
+            return "b"
+    except TypeError:
+        pass
+    return "s"
 
 
 class ArgPack:

@@ -11,6 +11,7 @@
 
 #include "../../include/isvd.h"
 #include "la.hpp"
+#include "mixed.hpp"
 #include "stages.hpp"
 
 size_t isvd_gpu_memory_limit = 0;
@@ -162,12 +163,36 @@ static bool guarded(F &&body) {
 }
 
 // ---------------------------------------------------------------------------------------------
+// precision families: s and d (the reference's), b (A and Omega in BF16, the rest FP32)
+// ---------------------------------------------------------------------------------------------
+struct PrecS {
+  using TA = float;
+  using T = float;
+  static constexpr bool mixed = false;
+  static constexpr const char *name = "Single Precision";
+};
+struct PrecD {
+  using TA = double;
+  using T = double;
+  static constexpr bool mixed = false;
+  static constexpr const char *name = "Double Precision";
+};
+struct PrecB {
+  using TA = uint16_t;
+  using T = float;
+  static constexpr bool mixed = true;
+  static constexpr const char *name = "Mixed Precision (BF16 A)";
+};
+
+// ---------------------------------------------------------------------------------------------
 // stage entry points (templated, wrapped by the extern "C" functions below)
 // ---------------------------------------------------------------------------------------------
-template <typename T>
-static void sketch_impl(const isvd_Param &param, char dista, char ordera, const T *a, i64 lda,
-                        const T *omegat, i64 ldo, T *yst, i64 ldyst, bool use_seed,
-                        unsigned long long seed) {
+template <class P>
+static void sketch_impl(const isvd_Param &param, char dista, char ordera,
+                        const typename P::TA *a, i64 lda, const typename P::TA *omegat, i64 ldo,
+                        typename P::T *yst, i64 ldyst, bool use_seed, unsigned long long seed) {
+  using T = typename P::T;
+  using TA = typename P::TA;
   const char dista_ = arg2char("DISTA", dista, "CR");
   const char ordera_ = arg2char("ORDERA", ordera, "CR");
   if (!dista_ || !ordera_) return;
@@ -181,30 +206,31 @@ static void sketch_impl(const isvd_Param &param, char dista, char ordera, const 
     ISVD_CHECK(ldyst >= d.Nl, "ldyst (%lld) must be >= Nl (%lld)", ldyst, d.Nl);
   ensure_init();
   const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
-  Arg<T> A(a, a_span, true, false);
+  Arg<TA> A(a, a_span, true, false);
   Arg<T> Y(yst, span(d.Nl, d.m, ldyst), false, true);
-  DBuf<T> om;
-  const T *omd;
+  DBuf<TA> om;
+  const TA *omd;
+  if (!use_seed) ISVD_CHECK(ldo >= d.Nl, "ldomegat (%lld) must be >= Nl (%lld)", ldo, d.Nl);
+  Arg<TA> O(use_seed ? nullptr : omegat, use_seed ? 0 : span(d.Nl, d.n, ldo), true, false);
   if (use_seed) {
     om.alloc(d.Nl * d.n);
-    gaussian_omega<T>(d.Nl, d.n, om.get(), d.Nl, seed);
+    if constexpr (P::mixed) gaussian_omega_b(d.Nl, d.n, om.get(), d.Nl, seed);
+    else gaussian_omega<T>(d.Nl, d.n, om.get(), d.Nl, seed);
     omd = om.get();
     ldo = d.Nl;
   } else {
-    ISVD_CHECK(ldo >= d.Nl, "ldomegat (%lld) must be >= Nl (%lld)", ldo, d.Nl);
-    Arg<T> O(omegat, span(d.Nl, d.n, ldo), true, false);
-    stage_sketch<T>(d, ordera_, A.dev, lda, O.dev, ldo, Y.dev, ldyst);
-    Y.finish();
-    sync();
-    return;
+    omd = O.dev;
   }
-  stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, Y.dev, ldyst);
+  if constexpr (P::mixed) stage_sketch_b(d, ordera_, A.dev, lda, omd, ldo, Y.dev, ldyst);
+  else stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, Y.dev, ldyst);
   Y.finish();
   sync();
 }
 
-template <typename T>
-static void orth_impl(const isvd_Param &param, i64 argc, T *yst, i64 ldyst, bool tsqr) {
+template <class P>
+static void orth_impl(const isvd_Param &param, i64 argc, typename P::T *yst, i64 ldyst,
+                      bool tsqr) {
+  using T = typename P::T;
   if (argc < 0) return;
   check_param(param);
   const Dims d = dims_of(param);
@@ -213,17 +239,22 @@ static void orth_impl(const isvd_Param &param, i64 argc, T *yst, i64 ldyst, bool
   if (tsqr) ISVD_CHECK(d.m >= d.l, "tall-skinny QR needs m >= l");
   ensure_init();
   Arg<T> Y(yst, span(d.Nl, d.m, ldyst), true, true);
-  if (tsqr)
-    stage_orth_tsqr<T>(d, Y.dev, ldyst);
-  else
-    stage_orth_gramian<T>(d, Y.dev, ldyst);
+  if constexpr (P::mixed) {
+    if (tsqr) stage_orth_tsqr_b(d, Y.dev, ldyst);
+    else stage_orth_gramian_b(d, Y.dev, ldyst);
+  } else {
+    if (tsqr) stage_orth_tsqr<T>(d, Y.dev, ldyst);
+    else stage_orth_gramian<T>(d, Y.dev, ldyst);
+  }
   Y.finish();
   sync();
 }
 
-template <typename T>
-static void kn_impl(const isvd_Param &param, const T *argv, i64 argc, T *retv, i64 retc,
-                    const T *yst, i64 ldyst, T *qt, i64 ldqt) {
+template <class P>
+static void kn_impl(const isvd_Param &param, const typename P::T *argv, i64 argc,
+                    typename P::T *retv, i64 retc, const typename P::T *yst, i64 ldyst,
+                    typename P::T *qt, i64 ldqt) {
+  using T = typename P::T;
   KnArgs ka;
   // query (@x@_integrate_kolmogorov_nagumo.c:56-61)
   if (argc < 0) {
@@ -246,16 +277,19 @@ static void kn_impl(const isvd_Param &param, const T *argv, i64 argc, T *retv, i
   Arg<T> Q(qt, span(d.l, d.m, ldqt), false, true);
   i64 it = 0;
   double err = 0;
-  stage_integrate_kn<T>(d, ka, Y.dev, ldyst, Q.dev, ldqt, &it, &err);
+  if constexpr (P::mixed) stage_integrate_kn_b(d, ka, Y.dev, ldyst, Q.dev, ldqt, &it, &err);
+  else stage_integrate_kn<T>(d, ka, Y.dev, ldyst, Q.dev, ldqt, &it, &err);
   Q.finish();
   sync();
   if (retc > 0) retv[0] = (T)it;
   if (retc > 1) retv[1] = (T)err;
 }
 
-template <typename T>
-static void wy_impl(const isvd_Param &param, const T *argv, i64 argc, T *retv, i64 retc,
-                    const T *yst, i64 ldyst, T *qt, i64 ldqt) {
+template <class P>
+static void wy_impl(const isvd_Param &param, const typename P::T *argv, i64 argc,
+                    typename P::T *retv, i64 retc, const typename P::T *yst, i64 ldyst,
+                    typename P::T *qt, i64 ldqt) {
+  using T = typename P::T;
   WyArgs wa;
   const double defaults[9] = {(double)wa.maxit, wa.tol, wa.tau0, wa.taumax, wa.taumin,
                               (double)wa.taumaxit, wa.beta, wa.sigma, wa.eta};
@@ -286,17 +320,21 @@ static void wy_impl(const isvd_Param &param, const T *argv, i64 argc, T *retv, i
   Arg<T> Q(qt, span(d.l, d.m, ldqt), false, true);
   i64 it = 0;
   double mu = 0;
-  stage_integrate_wy<T>(d, wa, Y.dev, ldyst, Q.dev, ldqt, &it, &mu);
+  if constexpr (P::mixed) stage_integrate_wy_b(d, wa, Y.dev, ldyst, Q.dev, ldqt, &it, &mu);
+  else stage_integrate_wy<T>(d, wa, Y.dev, ldyst, Q.dev, ldqt, &it, &mu);
   Q.finish();
   sync();
   if (retc > 0) retv[0] = (T)it;
   if (retc > 1) retv[1] = (T)mu;
 }
 
-template <typename T>
-static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera, const T *a,
-                      i64 lda, const T *qt, i64 ldqt, T *s, T *ut, i64 ldut, T *vt, i64 ldvt,
-                      int ut_root, int vt_root, bool tsqr) {
+template <class P>
+static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera,
+                      const typename P::TA *a, i64 lda, const typename P::T *qt, i64 ldqt,
+                      typename P::T *s, typename P::T *ut, i64 ldut, typename P::T *vt,
+                      i64 ldvt, int ut_root, int vt_root, bool tsqr) {
+  using T = typename P::T;
+  using TA = typename P::TA;
   if (argc < 0) return;
   const char dista_ = arg2char("DISTA", dista, "CR");
   const char ordera_ = arg2char("ORDERA", ordera, "CR");
@@ -317,12 +355,15 @@ static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera
   if (want_v) ISVD_CHECK(vt != nullptr, "vt is NULL");
   ensure_init();
   const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
-  Arg<T> A(a, a_span, true, false);
+  Arg<TA> A(a, a_span, true, false);
   Arg<T> Q(qt, span(d.l, d.m, ldqt), true, false);
   Arg<T> S(s, d.l, false, true);
   Arg<T> U(want_u ? ut : nullptr, span(d.k, d.m, ldut), false, true);
   Arg<T> V(want_v ? vt : nullptr, span(d.k, d.n, ldvt), false, true);
-  if (tsqr)
+  if constexpr (P::mixed)
+    stage_post_b(d, tsqr, ordera_, A.dev, lda, Q.dev, ldqt, S.dev, U.dev, ldut, V.dev, ldvt,
+                 want_u, want_v);
+  else if (tsqr)
     stage_post_tsqr<T>(d, ordera_, A.dev, lda, Q.dev, ldqt, S.dev, U.dev, ldut, V.dev, ldvt,
                        want_u, want_v);
   else
@@ -342,25 +383,28 @@ static const char *const kOptsO[] = {"TS", "GR"};
 static const char *const kOptsI[] = {"KN", "WY"};
 static const char *const kOptsP[] = {"TS", "GR", "TS_gpu", "GR_gpu"};
 
-static const char *alg_name(char kind, const char *alg, bool dbl) {
+static const char *alg_name(char kind, const char *alg, const char *prec) {
   static thread_local std::string s;
-  const char *prec = dbl ? "Double" : "Single";
   std::string name;
   if (kind == 'S') name = "Gaussian Projection Sketching";
   else if (kind == 'O') name = (alg[0] == 'T') ? "Tall-Skinny QR Orthogonalization" : "Gramian Orthogonalization";
   else if (kind == 'I') name = (alg[0] == 'K') ? "Kolmogorov-Nagumo Integration" : "Wen-Yin Integration";
   else name = (alg[0] == 'T') ? "Tall-Skinny QR Postprocessing" : "Gramian Postprocessing";
-  s = std::string(prec) + " Precision GPU (MI355X) " + name;
+  s = std::string(prec) + " GPU (MI355X) " + name;
   return s.c_str();
 }
 
-template <typename T>
+template <class P>
 static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *alg_o,
-                      const char *alg_i, const char *alg_p, const T *argv[4], const isvd_int_t argc[4],
-                      T *retv[4], const isvd_int_t retc[4], double time[4], FILE *stream,
-                      char dista, char ordera, const T *a, i64 lda, const T *omegat, i64 ldo,
-                      T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool use_seed,
-                      unsigned long long seed, int ut_root, int vt_root) {
+                      const char *alg_i, const char *alg_p, const typename P::T *argv[4],
+                      const isvd_int_t argc[4], typename P::T *retv[4], const isvd_int_t retc[4],
+                      double time[4], FILE *stream, char dista, char ordera,
+                      const typename P::TA *a, i64 lda, const typename P::TA *omegat, i64 ldo,
+                      typename P::T *s, typename P::T *ut, i64 ldut, typename P::T *vt,
+                      i64 ldvt, bool use_seed, unsigned long long seed, int ut_root,
+                      int vt_root) {
+  using T = typename P::T;
+  using TA = typename P::TA;
   const char *as = arg2str("ALG_S", alg_s, kOptsS, 2);
   const char *ao = arg2str("ALG_O", alg_o, kOptsO, 2);
   const char *ai = arg2str("ALG_I", alg_i, kOptsI, 2);
@@ -377,8 +421,8 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   // query mode (@x@_isvd.c:199-218)
   if (argc_s < 0 || argc_o < 0 || argc_i < 0 || argc_p < 0) {
     if (argc_i < 0) {
-      if (kn) kn_impl<T>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
-      else wy_impl<T>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
+      if (kn) kn_impl<P>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
+      else wy_impl<P>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
     }
     return;
   }
@@ -387,12 +431,11 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   if (!dista_ || !ordera_) return;
   check_param(param);
   const Dims d = dims_of(param);
-  const bool dbl = sizeof(T) == 8;
   if (stream) {
-    std::fprintf(stream, "Using %s.\n", alg_name('S', as, dbl));
-    std::fprintf(stream, "Using %s.\n", alg_name('O', ao, dbl));
-    std::fprintf(stream, "Using %s.\n", alg_name('I', ai, dbl));
-    std::fprintf(stream, "Using %s.\n\n", alg_name('P', ap, dbl));
+    std::fprintf(stream, "Using %s.\n", alg_name('S', as, P::name));
+    std::fprintf(stream, "Using %s.\n", alg_name('O', ao, P::name));
+    std::fprintf(stream, "Using %s.\n", alg_name('I', ai, P::name));
+    std::fprintf(stream, "Using %s.\n\n", alg_name('P', ap, P::name));
   }
   // argument checks of every stage, before any device work
   ISVD_CHECK(lda >= (ordera_ == 'C' ? d.m : d.n), "lda (%lld) too small", lda);
@@ -425,11 +468,12 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   (void)argc_s; (void)argc_o; (void)argc_p;
   ensure_init();
   const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
-  Arg<T> A(a, a_span, true, false);
+  Arg<TA> A(a, a_span, true, false);
   Arg<T> S(s, d.l, false, true);
   Arg<T> U(want_u ? ut : nullptr, span(d.k, d.m, ldut), false, true);
   Arg<T> V(want_v ? vt : nullptr, span(d.k, d.n, ldvt), false, true);
-  DBuf<T> yst(d.Nl * d.m), qt(d.l * d.m), om;
+  DBuf<T> yst(d.Nl * d.m), qt(d.l * d.m);
+  DBuf<TA> om;
   hipEvent_t ev[5];
   for (auto &e : ev) HIP_CHECK(hipEventCreate(&e));
   auto stage_msg = [&](const char *msg) {
@@ -440,28 +484,43 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   // sketching (@x@_isvd.c:243-247)
   stage_msg("Sketching ..................... ");
   HIP_CHECK(hipEventRecord(ev[0], ctx().stream));
-  const T *omd = omegat;
-  Arg<T> O(use_seed ? nullptr : omegat, use_seed ? 0 : span(d.Nl, d.n, ldo), true, false);
+  const TA *omd = omegat;
+  Arg<TA> O(use_seed ? nullptr : omegat, use_seed ? 0 : span(d.Nl, d.n, ldo), true, false);
   if (use_seed) {
     om.alloc(d.Nl * d.n);
-    gaussian_omega<T>(d.Nl, d.n, om.get(), d.Nl, seed);
+    if constexpr (P::mixed) gaussian_omega_b(d.Nl, d.n, om.get(), d.Nl, seed);
+    else gaussian_omega<T>(d.Nl, d.n, om.get(), d.Nl, seed);
     omd = om.get();
     ldo = d.Nl;
   } else {
     omd = O.dev;
   }
-  stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
+  if constexpr (P::mixed) stage_sketch_b(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
+  else stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
   HIP_CHECK(hipEventRecord(ev[1], ctx().stream));
   stage_msg("done.\nOrthogonalizing ............... ");
-  if (ao[0] == 'T') stage_orth_tsqr<T>(d, yst.get(), d.Nl);
-  else stage_orth_gramian<T>(d, yst.get(), d.Nl);
+  if constexpr (P::mixed) {
+    if (ao[0] == 'T') stage_orth_tsqr_b(d, yst.get(), d.Nl);
+    else stage_orth_gramian_b(d, yst.get(), d.Nl);
+  } else {
+    if (ao[0] == 'T') stage_orth_tsqr<T>(d, yst.get(), d.Nl);
+    else stage_orth_gramian<T>(d, yst.get(), d.Nl);
+  }
   HIP_CHECK(hipEventRecord(ev[2], ctx().stream));
   stage_msg("done.\nIntegrating ................... ");
-  if (kn) stage_integrate_kn<T>(d, ka, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
-  else stage_integrate_wy<T>(d, wa, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
+  if constexpr (P::mixed) {
+    if (kn) stage_integrate_kn_b(d, ka, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
+    else stage_integrate_wy_b(d, wa, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
+  } else {
+    if (kn) stage_integrate_kn<T>(d, ka, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
+    else stage_integrate_wy<T>(d, wa, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
+  }
   HIP_CHECK(hipEventRecord(ev[3], ctx().stream));
   stage_msg("done.\nPostprocessing ................ ");
-  if (ap[0] == 'T')
+  if constexpr (P::mixed)
+    stage_post_b(d, ap[0] == 'T', ordera_, A.dev, lda, qt.get(), d.l, S.dev, U.dev, ldut, V.dev,
+                 ldvt, want_u, want_v);
+  else if (ap[0] == 'T')
     stage_post_tsqr<T>(d, ordera_, A.dev, lda, qt.get(), d.l, S.dev, U.dev, ldut, V.dev, ldvt,
                        want_u, want_v);
   else
@@ -562,16 +621,16 @@ isvd_d_val_t *isvd_dMalloc(const size_t num) {
 }
 void isvd_Free(void *ptr) { std::free(ptr); }
 
-#define ISVD_DEFINE(x, T)                                                                      \
+#define ISVD_DEFINE(x, P, T, TA)                                                               \
   void isvd_##x##Isvd(const isvd_Param param, const char *alg_s, const char *alg_o,           \
                       const char *alg_i, const char *alg_p, const T *argv[4],                 \
                       const isvd_int_t argc[4], T *retv[4], const isvd_int_t retc[4],         \
                       double time[4], FILE *stream, const char dista, const char ordera,      \
-                      const T *a, const isvd_int_t lda, T *s, T *ut, const isvd_int_t ldut,   \
+                      const TA *a, const isvd_int_t lda, T *s, T *ut, const isvd_int_t ldut,   \
                       T *vt, const isvd_int_t ldvt, const isvd_int_t seed,                    \
                       const mpi_int_t ut_root, const mpi_int_t vt_root) {                     \
     guarded([&] {                                                                             \
-      isvd_impl<T>(param, alg_s, alg_o, alg_i, alg_p, argv, argc, retv, retc, time, stream,   \
+      isvd_impl<P>(param, alg_s, alg_o, alg_i, alg_p, argv, argc, retv, retc, time, stream,   \
                    dista, ordera, a, lda, nullptr, 0, s, ut, ldut, vt, ldvt, true,            \
                    (unsigned long long)(long long)seed, ut_root, vt_root);                    \
     });                                                                                       \
@@ -580,42 +639,34 @@ void isvd_Free(void *ptr) { std::free(ptr); }
                            const char *alg_i, const char *alg_p, const T *argv[4],            \
                            const isvd_int_t argc[4], T *retv[4], const isvd_int_t retc[4],    \
                            double time[4], FILE *stream, const char dista, const char ordera, \
-                           const T *a, const isvd_int_t lda, const T *omegat,                 \
+                           const TA *a, const isvd_int_t lda, const TA *omegat,                 \
                            const isvd_int_t ldomegat, T *s, T *ut, const isvd_int_t ldut,     \
                            T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,             \
                            const mpi_int_t vt_root) {                                         \
     guarded([&] {                                                                             \
-      isvd_impl<T>(param, alg_s, alg_o, alg_i, alg_p, argv, argc, retv, retc, time, stream,   \
+      isvd_impl<P>(param, alg_s, alg_o, alg_i, alg_p, argv, argc, retv, retc, time, stream,   \
                    dista, ordera, a, lda, omegat, ldomegat, s, ut, ldut, vt, ldvt, false, 0,  \
                    ut_root, vt_root);                                                         \
     });                                                                                       \
   }                                                                                           \
   void isvd_##x##SketchGaussianProjection(                                                     \
       const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
-      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t retc, const char dista, const char ordera, const TA *a,                 \
       const isvd_int_t lda, T *yst, const isvd_int_t ldyst, const isvd_int_t seed,            \
       const mpi_int_t mpi_root) {                                                             \
     (void)argv; (void)retv; (void)retc; (void)mpi_root;                                       \
     if (argc < 0) return;                                                                     \
     guarded([&] {                                                                             \
-      sketch_impl<T>(param, dista, ordera, a, lda, nullptr, 0, yst, ldyst, true,              \
+      sketch_impl<P>(param, dista, ordera, a, lda, nullptr, 0, yst, ldyst, true,              \
                      (unsigned long long)(long long)seed);                                    \
     });                                                                                       \
   }                                                                                           \
-  void isvd_##x##SketchGaussianProjection_gpu(                                                 \
-      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
-      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
-      const isvd_int_t lda, T *yst, const isvd_int_t ldyst, const isvd_int_t seed,            \
-      const mpi_int_t mpi_root) {                                                             \
-    isvd_##x##SketchGaussianProjection(param, argv, argc, retv, retc, dista, ordera, a, lda,  \
-                                       yst, ldyst, seed, mpi_root);                           \
-  }                                                                                           \
   void isvd_##x##SketchGaussianProjectionOmega(                                                \
-      const isvd_Param param, const char dista, const char ordera, const T *a,                \
-      const isvd_int_t lda, const T *omegat, const isvd_int_t ldomegat, T *yst,               \
+      const isvd_Param param, const char dista, const char ordera, const TA *a,                \
+      const isvd_int_t lda, const TA *omegat, const isvd_int_t ldomegat, T *yst,               \
       const isvd_int_t ldyst) {                                                               \
     guarded([&] {                                                                             \
-      sketch_impl<T>(param, dista, ordera, a, lda, omegat, ldomegat, yst, ldyst, false, 0);   \
+      sketch_impl<P>(param, dista, ordera, a, lda, omegat, ldomegat, yst, ldyst, false, 0);   \
     });                                                                                       \
   }                                                                                           \
   void isvd_##x##OrthogonalizeTallSkinnyQr(const isvd_Param param, const T *argv,             \
@@ -623,49 +674,59 @@ void isvd_Free(void *ptr) { std::free(ptr); }
                                            const isvd_int_t retc, T *yst,                     \
                                            const isvd_int_t ldyst) {                          \
     (void)argv; (void)retv; (void)retc;                                                       \
-    guarded([&] { orth_impl<T>(param, argc, yst, ldyst, true); });                            \
+    guarded([&] { orth_impl<P>(param, argc, yst, ldyst, true); });                            \
   }                                                                                           \
   void isvd_##x##OrthogonalizeGramian(const isvd_Param param, const T *argv,                  \
                                       const isvd_int_t argc, T *retv, const isvd_int_t retc,  \
                                       T *yst, const isvd_int_t ldyst) {                       \
     (void)argv; (void)retv; (void)retc;                                                       \
-    guarded([&] { orth_impl<T>(param, argc, yst, ldyst, false); });                           \
+    guarded([&] { orth_impl<P>(param, argc, yst, ldyst, false); });                           \
   }                                                                                           \
   void isvd_##x##IntegrateKolmogorovNagumo(const isvd_Param param, const T *argv,             \
                                            const isvd_int_t argc, T *retv,                    \
                                            const isvd_int_t retc, const T *yst,               \
                                            const isvd_int_t ldyst, T *qt,                     \
                                            const isvd_int_t ldqt) {                           \
-    guarded([&] { kn_impl<T>(param, argv, argc, retv, retc, yst, ldyst, qt, ldqt); });        \
+    guarded([&] { kn_impl<P>(param, argv, argc, retv, retc, yst, ldyst, qt, ldqt); });        \
   }                                                                                           \
   void isvd_##x##IntegrateWenYin(const isvd_Param param, const T *argv, const isvd_int_t argc, \
                                  T *retv, const isvd_int_t retc, const T *yst,                \
                                  const isvd_int_t ldyst, T *qt, const isvd_int_t ldqt) {      \
-    guarded([&] { wy_impl<T>(param, argv, argc, retv, retc, yst, ldyst, qt, ldqt); });        \
+    guarded([&] { wy_impl<P>(param, argv, argc, retv, retc, yst, ldyst, qt, ldqt); });        \
   }                                                                                           \
   void isvd_##x##PostprocessTallSkinnyQr(                                                      \
       const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
-      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t retc, const char dista, const char ordera, const TA *a,                 \
       const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
       const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
       const mpi_int_t vt_root) {                                                              \
     (void)argv; (void)retv; (void)retc;                                                       \
     guarded([&] {                                                                             \
-      post_impl<T>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
+      post_impl<P>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
                    ut_root, vt_root, true);                                                   \
     });                                                                                       \
   }                                                                                           \
   void isvd_##x##PostprocessGramian(                                                           \
       const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
-      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t retc, const char dista, const char ordera, const TA *a,                 \
       const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
       const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
       const mpi_int_t vt_root) {                                                              \
     (void)argv; (void)retv; (void)retc;                                                       \
     guarded([&] {                                                                             \
-      post_impl<T>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
+      post_impl<P>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
                    ut_root, vt_root, false);                                                  \
     });                                                                                       \
+  }
+
+#define ISVD_DEFINE_GPU_TWINS(x, T)                                                            \
+  void isvd_##x##SketchGaussianProjection_gpu(                                                 \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, T *yst, const isvd_int_t ldyst, const isvd_int_t seed,            \
+      const mpi_int_t mpi_root) {                                                             \
+    isvd_##x##SketchGaussianProjection(param, argv, argc, retv, retc, dista, ordera, a, lda,  \
+                                       yst, ldyst, seed, mpi_root);                           \
   }                                                                                           \
   void isvd_##x##PostprocessTallSkinnyQr_gpu(                                                  \
       const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
@@ -686,7 +747,11 @@ void isvd_Free(void *ptr) { std::free(ptr); }
                                  ldqt, s, ut, ldut, vt, ldvt, ut_root, vt_root);              \
   }
 
-ISVD_DEFINE(s, isvd_s_val_t)
-ISVD_DEFINE(d, isvd_d_val_t)
+
+ISVD_DEFINE(s, PrecS, isvd_s_val_t, isvd_s_val_t)
+ISVD_DEFINE(d, PrecD, isvd_d_val_t, isvd_d_val_t)
+ISVD_DEFINE(b, PrecB, float, isvd_b_val_t)
+ISVD_DEFINE_GPU_TWINS(s, isvd_s_val_t)
+ISVD_DEFINE_GPU_TWINS(d, isvd_d_val_t)
 
 }  // extern "C"

@@ -1,0 +1,136 @@
+// Mixed-precision iSVD stages (no counterpart in the reference; the FP64 stages they mirror are
+// cited at each function).
+#include "mixed.hpp"
+
+#include "la.hpp"
+#include "mx.hpp"
+
+namespace isvd {
+
+namespace {
+
+// U_b[:, j] /= sqrt(s_b[j])  (s: l values per problem, U: l x l per problem)
+__global__ void colscale_rsqrt_kernel(int l, const double *s, double *U) {
+  const i64 b = blockIdx.y;
+  const double *sb = s + b * l;
+  double *Ub = U + b * (i64)l * l;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < l * l; e += gridDim.x * blockDim.x)
+    Ub[e] /= sqrt(sb[e / l]);
+}
+
+int grid_for(i64 n) {
+  i64 b = (n + 255) / 256;
+  return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+// M := Qs^T Qs (Nl x Nl) from the FP32 bases by split-BF16 x3 products
+void gram_split(const Dims &d, const float *yst, i64 ldyst, float *m32) {
+  MxOperand q;
+  q.p = yst; q.ld = ldyst; q.f32 = true; q.kc = false; q.parts = 2;
+  gemm_mx(q, q, d.Nl, d.Nl, d.m, 1.f, 0.f, m32, d.Nl, 1, 0, true);
+}
+
+}  // namespace
+
+void gaussian_omega_b(i64 rows, i64 cols, uint16_t *omegat, i64 ld, unsigned long long seed) {
+  ISVD_CHECK(ld == rows, "gaussian_omega_b: ld must equal rows");
+  DBuf<float> tmp(rows * cols);
+  gaussian_omega<float>(rows, cols, tmp.get(), rows, seed);
+  f32_to_bf16(rows * cols, tmp.get(), omegat);
+}
+
+// sketchBlockRow (src/lib/libisvd/core/stage/@x@_sketch.h:136-139) in BF16
+void stage_sketch_b(const Dims &d, char ordera, const uint16_t *a, i64 lda,
+                    const uint16_t *omegat, i64 ldo, float *yst, i64 ldyst) {
+  MxOperand om, am;
+  om.p = omegat; om.ld = ldo; om.f32 = false; om.kc = false;
+  // op(A)^T(kk, j) = A(j, kk): column-major A has j contiguous, row-major A has kk contiguous
+  am.p = a; am.ld = lda; am.f32 = false; am.kc = (ordera == 'R');
+  gemm_mx(om, am, d.Nl, d.m, d.n, 1.f, 0.f, yst, ldyst);
+}
+
+// @x@_orthogonalize_gramian.c:81-98, twice
+void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst) {
+  const i64 l = d.l, N = d.N, m = d.m;
+  DBuf<float> w32(l * l * N), u32(l * l * N), ycopy(ldyst * m);
+  DBuf<double> w64(l * l * N), u64(l * l * N), s64(l * N);
+  for (int pass = 0; pass < 2; ++pass) {
+    // W_i := Y_i^T Y_i
+    gemm<float>('N', 'T', l, l, m, 1.f, yst, ldyst, yst, ldyst, 0.f, w32.get(), l, N, l, l, l * l);
+    convert<double, float>(l, l * N, w32.get(), l, w64.get(), l);
+    // W_i = U_i diag(s_i) U_i^T; U_i := U_i diag(s_i)^{-1/2}
+    svd_small<double>(l, w64.get(), l, l * l, s64.get(), l, u64.get(), l, l * l, nullptr, 0, 0, N);
+    hipLaunchKernelGGL(colscale_rsqrt_kernel, dim3(grid_for(l * l), (unsigned)N), dim3(256), 0,
+                       ctx().stream, (int)l, (const double *)s64.get(), u64.get());
+    LAUNCH_CHECK();
+    convert<float, double>(l, l * N, u64.get(), l, u32.get(), l);
+    // Q_i^T := U_i^T Y_i^T
+    HIP_CHECK(hipMemcpyAsync(ycopy.get(), yst, sizeof(float) * ldyst * m,
+                             hipMemcpyDeviceToDevice, ctx().stream));
+    gemm<float>('T', 'N', l, m, l, 1.f, u32.get(), l, ycopy.get(), ldyst, 0.f, yst, ldyst, N,
+                l * l, l, l);
+  }
+}
+
+// @x@_orthogonalize_tall_skinny_qr.c:127-143 in FP32
+void stage_orth_tsqr_b(const Dims &d, float *yst, i64 ldyst) {
+  stage_orth_tsqr<float>(d, yst, ldyst);
+}
+
+// @x@_integrate_kolmogorov_nagumo.c:155-253
+void stage_integrate_kn_b(const Dims &d, const KnArgs &args, const float *yst, i64 ldyst,
+                          float *qt, i64 ldqt, i64 *iter, double *err) {
+  const i64 l = d.l, Nl = d.Nl;
+  DBuf<float> m32(Nl * Nl), x32(Nl * l);
+  DBuf<double> m64(Nl * Nl), x64(Nl * l);
+  gram_split(d, yst, ldyst, m32.get());
+  convert<double, float>(Nl, Nl, m32.get(), Nl, m64.get(), Nl);
+  m32.release();
+  integrate_kn_coords<double>(d, args, m64.get(), x64.get(), iter, err);
+  convert<float, double>(Nl, l, x64.get(), Nl, x32.get(), Nl);
+  gemm<float>('T', 'N', l, d.m, Nl, 1.f, x32.get(), Nl, yst, ldyst, 0.f, qt, ldqt);
+}
+
+// @x@_integrate_wen_yin.c:201-362
+void stage_integrate_wy_b(const Dims &d, const WyArgs &args, const float *yst, i64 ldyst,
+                          float *qt, i64 ldqt, i64 *iter, double *mu) {
+  const i64 l = d.l, Nl = d.Nl;
+  DBuf<float> m32(Nl * Nl), x32(Nl * l);
+  DBuf<double> m64(Nl * Nl), x64(Nl * l);
+  gram_split(d, yst, ldyst, m32.get());
+  convert<double, float>(Nl, Nl, m32.get(), Nl, m64.get(), Nl);
+  m32.release();
+  integrate_wy_coords<double>(d, args, m64.get(), x64.get(), iter, mu);
+  convert<float, double>(Nl, l, x64.get(), Nl, x32.get(), Nl);
+  gemm<float>('T', 'N', l, d.m, Nl, 1.f, x32.get(), Nl, yst, ldyst, 0.f, qt, ldqt);
+}
+
+// projectBlockRow (@x@_postprocess.h:154-157) + @x@_postprocess_gramian.c:135-185 /
+// @x@_postprocess_tall_skinny_qr.c:184-296
+void stage_post_b(const Dims &d, bool tsqr, char ordera, const uint16_t *a, i64 lda,
+                  const float *qt, i64 ldqt, float *s, float *ut, i64 ldut, float *vt, i64 ldvt,
+                  bool want_u, bool want_v) {
+  const i64 l = d.l, m = d.m, n = d.n, k = d.k;
+  DBuf<float> zt32(l * n);
+  MxOperand qa, ab;
+  qa.p = qt; qa.ld = ldqt; qa.f32 = true; qa.kc = false; qa.parts = 3;
+  // op(A)(kk, j): column-major A is contiguous along kk
+  ab.p = a; ab.ld = lda; ab.f32 = false; ab.kc = (ordera == 'C');
+  gemm_mx(qa, ab, l, n, m, 1.f, 0.f, zt32.get(), l);
+  DBuf<double> zt64(l * n), qt64(l * m), s64(l), ut64(want_u ? l * m : 0),
+      vt64(want_v ? l * n : 0);
+  convert<double, float>(l, n, zt32.get(), l, zt64.get(), l);
+  zt32.release();
+  convert<double, float>(l, m, qt, ldqt, qt64.get(), l);
+  if (tsqr)
+    post_tsqr_z<double>(d, qt64.get(), l, zt64.get(), s64.get(), ut64.get(), l, vt64.get(), l,
+                        want_u, want_v);
+  else
+    post_gramian_z<double>(d, qt64.get(), l, zt64.get(), s64.get(), ut64.get(), l, vt64.get(),
+                           l, want_u, want_v);
+  convert<float, double>(l, 1, s64.get(), l, s, l);
+  if (want_u) convert<float, double>(k, m, ut64.get(), l, ut, ldut);
+  if (want_v) convert<float, double>(k, n, vt64.get(), l, vt, ldvt);
+}
+
+}  // namespace isvd

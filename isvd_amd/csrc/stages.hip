@@ -248,24 +248,22 @@ void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst) {
 //   X_+ = X_c F + (B_c / N) C^-1   (the image of Q_+ = Q_c F + G_c C^-1),
 // so each iteration touches only Nl x l and l x l matrices; Qbar = Qs X is formed at the end.
 template <typename T>
-void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldyst, T *qt,
-                        i64 ldqt, i64 *iter_out, double *err_out) {
-  const i64 l = d.l, Nl = d.Nl, m = d.m;
+void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out, i64 *iter_out,
+                         double *err_out) {
+  const i64 l = d.l, Nl = d.Nl;
   const T ninv = T(1) / T(d.N);
-  DBuf<T> M(Nl * Nl), xc(Nl * l), xp(Nl * l), bc(Nl * l), bp(Nl * l), bgc(Nl * l);
+  DBuf<T> xc(Nl * l), xp(Nl * l), bc(Nl * l), bp(Nl * l), bgc(Nl * l);
   DBuf<T> dc(l * l), z(l * l), w(l), zv(l * l), zs(l * l), zi(l * l), c(l * l), cinv(l * l),
       err(1);
-  // M := Qs^T Qs
-  syrk_full<T>('N', Nl, m, T(1), yst, ldyst, M.get(), Nl);
   // X_c := [I; 0] (Qc := Q_0, :156);  B_c := Qs^T Q_0 = M[:, 0:l]  (:159-160)
   fill<T>(Nl, l, T(0), xc.get(), Nl, T(1));
-  omatcopy<T>('N', Nl, l, T(1), M.get(), Nl, bc.get(), Nl);
+  omatcopy<T>('N', Nl, l, T(1), M, Nl, bc.get(), Nl);
   i64 it;
   double error = -INFINITY;
   DBuf<T> work(6 * l * l), rho(1);
   for (it = 1;; ++it) {
     // Bgc := M Bc / N  (:174-178)
-    gemm<T>('N', 'N', Nl, l, Nl, ninv, M.get(), Nl, bc.get(), Nl, T(0), bgc.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, Nl, ninv, M, Nl, bc.get(), Nl, T(0), bgc.get(), Nl);
     // Dc := Bc^T Bc / N (:181); Z := Bc^T Bgc / N - Dc^T Dc (:184-190)
     gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bc.get(), Nl, T(0), dc.get(), l);
     gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bgc.get(), Nl, T(0), z.get(), l);
@@ -321,32 +319,42 @@ void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldy
     std::swap(xc.p, xp.p);
     std::swap(bc.p, bp.p);
   }
-  // Qbar^T := X_+^T Qs^T  (:250-253)
-  gemm<T>('T', 'N', l, m, Nl, T(1), xp.get(), Nl, yst, ldyst, T(0), qt, ldqt);
+  omatcopy<T>('N', Nl, l, T(1), xp.get(), Nl, x_out, Nl);
   *iter_out = it;
   *err_out = error;
+}
+
+template <typename T>
+void stage_integrate_kn(const Dims &d, const KnArgs &args, const T *yst, i64 ldyst, T *qt,
+                        i64 ldqt, i64 *iter_out, double *err_out) {
+  const i64 l = d.l, Nl = d.Nl, m = d.m;
+  DBuf<T> M(Nl * Nl), x(Nl * l);
+  // M := Qs^T Qs
+  syrk_full<T>('N', Nl, m, T(1), yst, ldyst, M.get(), Nl);
+  integrate_kn_coords<T>(d, args, M.get(), x.get(), iter_out, err_out);
+  // Qbar^T := X_+^T Qs^T  (:250-253)
+  gemm<T>('T', 'N', l, m, Nl, T(1), x.get(), Nl, yst, ldyst, T(0), qt, ldqt);
 }
 
 // Wen-Yin integration (@x@_integrate_wen_yin.c:201-384), same coordinates as KN:
 // Q_c = Qs xq, G_c = Qs gq (gq = B_c / N), X_c = Qs xx, <Qs a, Qs b> = <a, M b>; the
 // 2l x 2l inverse of the line search is replaced by its closed form (sym_scale_kernel).
 template <typename T>
-void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldyst, T *qt,
-                        i64 ldqt, i64 *iter_out, double *mu_out) {
-  const i64 l = d.l, Nl = d.Nl, m = d.m;
+void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out, i64 *iter_out,
+                         double *mu_out) {
+  const i64 l = d.l, Nl = d.Nl;
   const T ninv = T(1) / T(d.N);
   const T half_ninv = T(0.5) / T(d.N);
-  DBuf<T> M(Nl * Nl), xq(Nl * l), xqp(Nl * l), xx(Nl * l), xxp(Nl * l), bc(Nl * l), bp(Nl * l),
+  DBuf<T> xq(Nl * l), xqp(Nl * l), xx(Nl * l), xxp(Nl * l), bc(Nl * l), bp(Nl * l),
       bgc(Nl * l), gq(Nl * l), d1(Nl * l), d2(Nl * l), md1(Nl * l), md2(Nl * l);
   DBuf<T> dc(l * l), dgc(l * l), zm(l * l), si(l * l), fcb(l * l), fgcb(l * l), scal(4);
   DBuf<int> info(1);
-  syrk_full<T>('N', Nl, m, T(1), yst, ldyst, M.get(), Nl);
   // init (:201-235)
   fill<T>(Nl, l, T(0), xq.get(), Nl, T(1));
-  omatcopy<T>('N', Nl, l, T(1), M.get(), Nl, bc.get(), Nl);
+  omatcopy<T>('N', Nl, l, T(1), M, Nl, bc.get(), Nl);
   gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bc.get(), Nl, T(0), dc.get(), l);
   omatcopy<T>('N', Nl, l, ninv, bc.get(), Nl, gq.get(), Nl);
-  gemm<T>('N', 'N', Nl, l, Nl, T(1), M.get(), Nl, gq.get(), Nl, T(0), bgc.get(), Nl);
+  gemm<T>('N', 'N', Nl, l, Nl, T(1), M, Nl, gq.get(), Nl, T(0), bgc.get(), Nl);
   gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bgc.get(), Nl, T(0), dgc.get(), l);
   omatcopy<T>('N', Nl, l, T(1), gq.get(), Nl, xx.get(), Nl);
   gemm<T>('N', 'N', Nl, l, l, T(-1), xq.get(), Nl, dc.get(), l, T(1), xx.get(), Nl);
@@ -404,7 +412,7 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
     // D+ := B+^T B+ / N (:295); G+ := Qs B+ / N (:298); Bg+ := Qs^T G+ (:301-303)
     gemm<T>('T', 'N', l, l, Nl, ninv, bp.get(), Nl, bp.get(), Nl, T(0), dc.get(), l);
     omatcopy<T>('N', Nl, l, ninv, bp.get(), Nl, gq.get(), Nl);
-    gemm<T>('N', 'N', Nl, l, Nl, T(1), M.get(), Nl, gq.get(), Nl, T(0), bgc.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, Nl, T(1), M, Nl, gq.get(), Nl, T(0), bgc.get(), Nl);
     // Dg+ := B+^T Bg+ / N (:306); mu (:307)
     gemm<T>('T', 'N', l, l, Nl, ninv, bp.get(), Nl, bgc.get(), Nl, T(0), dgc.get(), l);
     hipLaunchKernelGGL(wy_mu_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
@@ -427,8 +435,8 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
                        Nl * l, (const T *)xx.get(), (const T *)xxp.get(), d2.get());
     LAUNCH_CHECK();
     // t (:334-342) in sketch coordinates: <D_a, D_b> = <d_a, M d_b>
-    gemm<T>('N', 'N', Nl, l, Nl, T(1), M.get(), Nl, d1.get(), Nl, T(0), md1.get(), Nl);
-    gemm<T>('N', 'N', Nl, l, Nl, T(1), M.get(), Nl, d2.get(), Nl, T(0), md2.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, Nl, T(1), M, Nl, d1.get(), Nl, T(0), md1.get(), Nl);
+    gemm<T>('N', 'N', Nl, l, Nl, T(1), M, Nl, d2.get(), Nl, T(0), md2.get(), Nl);
     if (it % 2) {
       dot<T>(Nl * l, d1.get(), md1.get(), scal.get());
       dot<T>(Nl * l, d1.get(), md2.get(), scal.get() + 1);
@@ -446,10 +454,20 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
     std::swap(xx.p, xxp.p);
     std::swap(bc.p, bp.p);
   }
-  // Qbar^T := xq+^T Qs^T  (:359-362)
-  gemm<T>('T', 'N', l, m, Nl, T(1), xqp.get(), Nl, yst, ldyst, T(0), qt, ldqt);
+  omatcopy<T>('N', Nl, l, T(1), xqp.get(), Nl, x_out, Nl);
   *iter_out = it;
   *mu_out = mu;
+}
+
+template <typename T>
+void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldyst, T *qt,
+                        i64 ldqt, i64 *iter_out, double *mu_out) {
+  const i64 l = d.l, Nl = d.Nl, m = d.m;
+  DBuf<T> M(Nl * Nl), x(Nl * l);
+  syrk_full<T>('N', Nl, m, T(1), yst, ldyst, M.get(), Nl);
+  integrate_wy_coords<T>(d, args, M.get(), x.get(), iter_out, mu_out);
+  // Qbar^T := xq+^T Qs^T  (:359-362)
+  gemm<T>('T', 'N', l, m, Nl, T(1), x.get(), Nl, yst, ldyst, T(0), qt, ldqt);
 }
 
 // Z^T := Qbar^T op(A)  (projectBlockRow, src/lib/libisvd/core/stage/@x@_postprocess.h:154-157)
@@ -462,13 +480,12 @@ static void project(const Dims &d, char ordera, const T *a, i64 lda, const T *qt
 
 // Gramian postprocessing (src/lib/libisvd/core/stage/@x@_postprocess_gramian.c:135-185).
 template <typename T>
-void stage_post_gramian(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
-                        T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v) {
+void post_gramian_z(const Dims &d, const T *qt, i64 ldqt, T *zt, T *s, T *ut, i64 ldut, T *vt,
+                    i64 ldvt, bool want_u, bool want_v) {
   const i64 l = d.l, k = d.k;
-  DBuf<T> zt(l * d.n), w(l * l), uw(l * l);
-  project<T>(d, ordera, a, lda, qt, ldqt, zt.get());
+  DBuf<T> w(l * l), uw(l * l);
   // W := Z^T Z (:145); gesvd('O','N') + vSqrt (:149-151)
-  gemm<T>('N', 'T', l, l, d.n, T(1), zt.get(), l, zt.get(), l, T(0), w.get(), l);
+  gemm<T>('N', 'T', l, l, d.n, T(1), zt, l, zt, l, T(0), w.get(), l);
   svd_small<T>(l, w.get(), l, 0, s, l, uw.get(), l, 0, nullptr, 0, 0, 1);
   hipLaunchKernelGGL(sqrt_inplace_kernel<T>, dim3(1, 1), dim3(256), 0, ctx().stream, (int)l, s);
   LAUNCH_CHECK();
@@ -479,8 +496,16 @@ void stage_post_gramian(const Dims &d, char ordera, const T *a, i64 lda, const T
     hipLaunchKernelGGL(colscale_inv_kernel<T>, dim3(grid_for(l * k)), dim3(256), 0, ctx().stream,
                        (int)l, (int)k, (const T *)s, uw.get(), l);
     LAUNCH_CHECK();
-    gemm<T>('T', 'N', k, d.n, l, T(1), uw.get(), l, zt.get(), l, T(0), vt, ldvt);
+    gemm<T>('T', 'N', k, d.n, l, T(1), uw.get(), l, zt, l, T(0), vt, ldvt);
   }
+}
+
+template <typename T>
+void stage_post_gramian(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                        T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v) {
+  DBuf<T> zt(d.l * d.n);
+  project<T>(d, ordera, a, lda, qt, ldqt, zt.get());
+  post_gramian_z<T>(d, qt, ldqt, zt.get(), s, ut, ldut, vt, ldvt, want_u, want_v);
 }
 
 // Tall-skinny-QR postprocessing (src/lib/libisvd/core/stage/@x@_postprocess_tall_skinny_qr.c:
@@ -490,14 +515,13 @@ void stage_post_gramian(const Dims &d, char ordera, const T *a, i64 lda, const T
 // above only in projector form and only when k == l, and for k < l its V even depends on the
 // sign convention of gesvd, so it is not reproduced.
 template <typename T>
-void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
-                     T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v) {
+void post_tsqr_z(const Dims &d, const T *qt, i64 ldqt, T *zt, T *s, T *ut, i64 ldut, T *vt,
+                 i64 ldvt, bool want_u, bool want_v) {
   const i64 l = d.l, k = d.k;
   ISVD_CHECK(d.n >= l, "tall-skinny QR postprocessing needs n >= l (n = %lld, l = %lld)", d.n, l);
-  DBuf<T> zt(l * d.n), r(l * l), lmat(l * l), ul(l * l), vl(l * l);
-  project<T>(d, ordera, a, lda, qt, ldqt, zt.get());
+  DBuf<T> r(l * l), lmat(l * l), ul(l * l), vl(l * l);
   // Z (row-major n x l in zt) = Q_z R  <=>  Z^T = R^T Q_z^T = L Q_lq   (:199-208)
-  tsqr_rowmajor<T>(d.n, l, zt.get(), l, 0, 1, r.get(), l, 0);
+  tsqr_rowmajor<T>(d.n, l, zt, l, 0, 1, r.get(), l, 0);
   hipLaunchKernelGGL(transpose_small_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
                      ctx().stream, (int)l, (const T *)r.get(), l, lmat.get(), l);
   LAUNCH_CHECK();
@@ -506,7 +530,15 @@ void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *q
   // U^T := U_L[:, :k]^T Qbar^T  (:263-265)
   if (want_u) gemm<T>('T', 'N', k, d.m, l, T(1), ul.get(), l, qt, ldqt, T(0), ut, ldut);
   // V^T := V_L[:, :k]^T Q_lq    (:278-284; Q_lq = Q_z^T is zt as an l x n col-major matrix)
-  if (want_v) gemm<T>('T', 'N', k, d.n, l, T(1), vl.get(), l, zt.get(), l, T(0), vt, ldvt);
+  if (want_v) gemm<T>('T', 'N', k, d.n, l, T(1), vl.get(), l, zt, l, T(0), vt, ldvt);
+}
+
+template <typename T>
+void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                     T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v) {
+  DBuf<T> zt(d.l * d.n);
+  project<T>(d, ordera, a, lda, qt, ldqt, zt.get());
+  post_tsqr_z<T>(d, qt, ldqt, zt.get(), s, ut, ldut, vt, ldvt, want_u, want_v);
 }
 
 #define INST(T)                                                                                \
@@ -521,7 +553,15 @@ void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *q
   template void stage_post_gramian<T>(const Dims &, char, const T *, i64, const T *, i64, T *, \
                                       T *, i64, T *, i64, bool, bool);                         \
   template void stage_post_tsqr<T>(const Dims &, char, const T *, i64, const T *, i64, T *,    \
-                                   T *, i64, T *, i64, bool, bool);
+                                   T *, i64, T *, i64, bool, bool);                            \
+  template void integrate_kn_coords<T>(const Dims &, const KnArgs &, const T *, T *, i64 *,    \
+                                       double *);                                              \
+  template void integrate_wy_coords<T>(const Dims &, const WyArgs &, const T *, T *, i64 *,    \
+                                       double *);                                              \
+  template void post_gramian_z<T>(const Dims &, const T *, i64, T *, T *, T *, i64, T *, i64,  \
+                                  bool, bool);                                                 \
+  template void post_tsqr_z<T>(const Dims &, const T *, i64, T *, T *, T *, i64, T *, i64,     \
+                               bool, bool);
 INST(double)
 INST(float)
 #undef INST

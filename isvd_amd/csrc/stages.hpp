@@ -55,4 +55,23 @@ template <typename T>
 void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
                      T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v);
 
+// The integration loops in the coordinates of the sketch basis, from M = Qs^T Qs (Nl x Nl):
+// x_out (Nl x l) receives the coefficients of Qbar = Qs x_out.
+template <typename T>
+void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out, i64 *iter,
+                         double *err);
+
+template <typename T>
+void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out, i64 *iter,
+                         double *mu);
+
+// Postprocessing from Z^T = Qbar^T op(A) (l x n, ld l), which the TSQR variant overwrites.
+template <typename T>
+void post_gramian_z(const Dims &d, const T *qt, i64 ldqt, T *zt, T *s, T *ut, i64 ldut, T *vt,
+                    i64 ldvt, bool want_u, bool want_v);
+
+template <typename T>
+void post_tsqr_z(const Dims &d, const T *qt, i64 ldqt, T *zt, T *s, T *ut, i64 ldut, T *vt,
+                 i64 ldvt, bool want_u, bool want_v);
+
 }  // namespace isvd

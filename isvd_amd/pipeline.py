@@ -21,10 +21,16 @@ def _is_torch(x):
     return hasattr(x, "data_ptr") and not isinstance(x, np.ndarray)
 
 
+def _family(x, mixed=False):
+    """(entry-point prefix, numpy dtype of the FP arrays) for an input array."""
+    if mixed:
+        return "b", np.float32
+    x_ = capi.prefix(x.dtype)
+    return x_, (np.float64 if x_ == "d" else np.float32)
+
+
 def _dtype(x):
-    if _is_torch(x):
-        return np.float64 if str(x.dtype) == "torch.float64" else np.float32
-    return x.dtype.type
+    return _family(x)[1]
 
 
 def _empty_like_kind(ref, shape, dtype):
@@ -49,8 +55,7 @@ def _a_ld(a, ordera):
 def sketch(param, a, omega=None, seed=None, ordera="R", dista="R"):
     """Y = A Omega.  With ``omega`` (n x Nl row-major) uses the caller's draw, else ``seed``."""
     lib = capi.load()
-    dt = _dtype(a)
-    x = capi.prefix(dt)
+    x, dt = _family(a)
     m, Nl = param.nrow, param.dim_sketch_total
     y = _empty_like_kind(a, (m, Nl), dt)
     if omega is not None:
@@ -66,21 +71,22 @@ def sketch(param, a, omega=None, seed=None, ordera="R", dista="R"):
     return y
 
 
-def orthogonalize(param, y, alg="GR"):
-    """In-place orthogonalization of every sketch block of Y (m x Nl)."""
+def orthogonalize(param, y, alg="GR", mixed=False):
+    """In-place orthogonalization of every sketch block of Y (m x Nl); ``mixed`` selects the
+    b family (FP32 Y, Gramian twice)."""
     lib = capi.load()
-    x = capi.prefix(_dtype(y))
+    x, _ = _family(y, mixed)
     name = {"GR": "OrthogonalizeGramian", "TS": "OrthogonalizeTallSkinnyQr"}[alg.upper()]
     getattr(lib, f"isvd_{x}{name}")(param, None, 0, None, 0, capi.addr(y), y.shape[1])
     _check(name)
     return y
 
 
-def integrate(param, qs, alg="KN", argv=None):
-    """Returns (Qbar m x l, iter, err_or_mu)."""
+def integrate(param, qs, alg="KN", argv=None, mixed=False):
+    """Returns (Qbar m x l, iter, err_or_mu); ``mixed`` selects the b family (FP32 bases,
+    split-BF16 Gram)."""
     lib = capi.load()
-    dt = _dtype(qs)
-    x = capi.prefix(dt)
+    x, dt = _family(qs, mixed)
     ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
     l = param.dim_sketch
     qbar = _empty_like_kind(qs, (param.nrow, l), dt)
@@ -98,8 +104,7 @@ def integrate(param, qs, alg="KN", argv=None):
 def postprocess(param, a, qbar, alg="GR", ordera="R", dista="R", want_u=True, want_v=True):
     """Returns (s[l], U m x l or None, V n x l or None); the first k columns of U/V are valid."""
     lib = capi.load()
-    dt = _dtype(a)
-    x = capi.prefix(dt)
+    x, dt = _family(a)
     l = param.dim_sketch
     s = _empty_like_kind(a, (l,), dt)
     u = _empty_like_kind(a, (param.nrow, l), dt) if want_u else None
@@ -115,10 +120,10 @@ def postprocess(param, a, qbar, alg="GR", ordera="R", dista="R", want_u=True, wa
 
 def isvd(param, a, omega=None, seed=0, algs=("GP", "GR", "KN", "GR"), ordera="R", dista="R",
          argv_i=None, want_u=True, want_v=True):
-    """Full driver (isvd_?Isvd / isvd_?IsvdOmega).  Returns dict(s, u, v, iter, err, time)."""
+    """Full driver (isvd_?Isvd / isvd_?IsvdOmega).  Returns dict(s, u, v, iter, err, time).
+    A BF16 ``a`` (torch.bfloat16 or uint16 bits) selects the mixed-precision b family."""
     lib = capi.load()
-    dt = _dtype(a)
-    x = capi.prefix(dt)
+    x, dt = _family(a)
     ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
     l = param.dim_sketch
     s = _empty_like_kind(a, (l,), dt)
