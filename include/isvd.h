@@ -214,6 +214,28 @@ ISVD_DECLARE_STAGES(s, isvd_s_val_t)
 ISVD_DECLARE_STAGES(d, isvd_d_val_t)
 
 /* =========================================================================================
+ * Adjoint of the driver's rank-k result (no counterpart in the reference; for callers that
+ * differentiate through the decomposition, e.g. a torch.autograd.Function).
+ * Given dL/dS (gs, k values), dL/dU^T (gut, k x m, ld >= k) and dL/dV^T (gvt, k x n,
+ * ld >= k) -- any of them NULL for zero -- writes dL/dA (da, same storage order as A,
+ * ldda >= m for ordera 'C', >= n for 'R').  U, S and V are recomputed from Qbar^T (qt, l x m,
+ * the output of the integration stage) by the postprocessing named by alg_p, as in the
+ * forward call.  With all l computed triplets:
+ *   dA = U [(F o (J - J^T)) S + S (F o (K - K^T)) + diag(gS)] V^T
+ *        + (I - U U^T) gU S^-1 V^T + U S^-1 gV^T (I - V V^T),
+ *   J = U^T gU, K = V^T gV, F_ij = 1 / (s_j^2 - s_i^2) for i != j (0 when the gap is below
+ *   1e-12 (double) / 1e-6 (single) of max(s_i^2, s_j^2)); gradients padded with zeros to l.
+ * ========================================================================================= */
+#define ISVD_DECLARE_BACKWARD(x, T)                                                            \
+  void isvd_##x##IsvdBackward(const isvd_Param param, const char *alg_p, const char dista,     \
+                              const char ordera, const T *a, const isvd_int_t lda,             \
+                              const T *qt, const isvd_int_t ldqt, const T *gs, const T *gut,   \
+                              const isvd_int_t ldgut, const T *gvt, const isvd_int_t ldgvt,    \
+                              T *da, const isvd_int_t ldda);
+ISVD_DECLARE_BACKWARD(s, isvd_s_val_t)
+ISVD_DECLARE_BACKWARD(d, isvd_d_val_t)
+
+/* =========================================================================================
  * Mixed precision, the "b" family (no counterpart in the reference; same argument order and
  * meaning as the d family above).  A and Omega are stored as BF16 (isvd_b_val_t holds the
  * bit pattern: the upper half of the FP32 value); every other array is FP32.

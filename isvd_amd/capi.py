@@ -89,6 +89,9 @@ def load(path=None):
         _sig(getattr(lib, f"isvd_{x}Isvd"), drv + [VP, VP, I, VP, I, I, mpi_int_t, mpi_int_t])
         _sig(getattr(lib, f"isvd_{x}IsvdOmega"),
              drv + [VP, I, VP, VP, I, VP, I, mpi_int_t, mpi_int_t])
+    for x in "sd":
+        _sig(getattr(lib, f"isvd_{x}IsvdBackward"),
+             [Param, ctypes.c_char_p, CH, CH, VP, I, VP, I, VP, VP, I, VP, I, VP, I])
     _lib = lib
     return lib
 

@@ -546,6 +546,47 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   }
 }
 
+// adjoint of the driver's rank-k output (no counterpart in the reference)
+template <class P>
+static void backward_impl(const isvd_Param &param, const char *alg_p, char dista, char ordera,
+                          const typename P::T *a, i64 lda, const typename P::T *qt, i64 ldqt,
+                          const typename P::T *gs, const typename P::T *gut, i64 ldgut,
+                          const typename P::T *gvt, i64 ldgvt, typename P::T *da, i64 ldda) {
+  using T = typename P::T;
+  const char *ap = arg2str("ALG_P", alg_p, kOptsP, 4);
+  const char dista_ = arg2char("DISTA", dista, "CR");
+  const char ordera_ = arg2char("ORDERA", ordera, "CR");
+  if (!ap || !dista_ || !ordera_) return;
+  check_param(param);
+  const Dims d = dims_of(param);
+  ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  ISVD_CHECK(d.n >= d.l || ap[0] != 'T', "tall-skinny QR postprocessing needs n >= l");
+  const i64 rows = ordera_ == 'C' ? d.m : d.n, cols = ordera_ == 'C' ? d.n : d.m;
+  ISVD_CHECK(lda >= rows, "lda (%lld) too small", lda);
+  ISVD_CHECK(ldda >= rows, "ldda (%lld) too small", ldda);
+  ISVD_CHECK(ldqt >= d.l, "ldqt (%lld) must be >= l", ldqt);
+  ISVD_CHECK(a != nullptr && qt != nullptr && da != nullptr, "a, qt and da must not be NULL");
+  if (gut) ISVD_CHECK(ldgut >= d.k, "ldgut (%lld) must be >= k", ldgut);
+  if (gvt) ISVD_CHECK(ldgvt >= d.k, "ldgvt (%lld) must be >= k", ldgvt);
+  ensure_init();
+  Arg<T> A(a, span(rows, cols, lda), true, false);
+  Arg<T> Q(qt, span(d.l, d.m, ldqt), true, false);
+  Arg<T> GS(gs, d.k, true, false);
+  Arg<T> GU(gut, gut ? span(d.k, d.m, ldgut) : 0, true, false);
+  Arg<T> GV(gvt, gvt ? span(d.k, d.n, ldgvt) : 0, true, false);
+  Arg<T> DA(da, span(rows, cols, ldda), false, true);
+  // absent gradients are zero
+  DBuf<T> zs, zu, zv;
+  const T *gs_d = GS.dev, *gu_d = GU.dev, *gv_d = GV.dev;
+  if (!gs_d) { zs.alloc(d.k); fill<T>(d.k, 1, T(0), zs.get(), d.k); gs_d = zs.get(); }
+  if (!gu_d) { zu.alloc(d.k * d.m); fill<T>(d.k, d.m, T(0), zu.get(), d.k); gu_d = zu.get(); ldgut = d.k; }
+  if (!gv_d) { zv.alloc(d.k * d.n); fill<T>(d.k, d.n, T(0), zv.get(), d.k); gv_d = zv.get(); ldgvt = d.k; }
+  stage_isvd_backward<T>(d, ap[0] == 'T', ordera_, A.dev, lda, Q.dev, ldqt, gs_d, gu_d, ldgut,
+                         gv_d, ldgvt, DA.dev, ldda);
+  DA.finish();
+  sync();
+}
+
 }  // namespace isvd
 
 using namespace isvd;
@@ -719,6 +760,18 @@ void isvd_Free(void *ptr) { std::free(ptr); }
     });                                                                                       \
   }
 
+#define ISVD_DEFINE_BACKWARD(x, P, T)                                                          \
+  void isvd_##x##IsvdBackward(const isvd_Param param, const char *alg_p, const char dista,     \
+                              const char ordera, const T *a, const isvd_int_t lda,             \
+                              const T *qt, const isvd_int_t ldqt, const T *gs, const T *gut,   \
+                              const isvd_int_t ldgut, const T *gvt, const isvd_int_t ldgvt,    \
+                              T *da, const isvd_int_t ldda) {                                  \
+    guarded([&] {                                                                             \
+      backward_impl<P>(param, alg_p, dista, ordera, a, lda, qt, ldqt, gs, gut, ldgut, gvt,    \
+                       ldgvt, da, ldda);                                                      \
+    });                                                                                       \
+  }
+
 #define ISVD_DEFINE_GPU_TWINS(x, T)                                                            \
   void isvd_##x##SketchGaussianProjection_gpu(                                                 \
       const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
@@ -753,5 +806,7 @@ ISVD_DEFINE(d, PrecD, isvd_d_val_t, isvd_d_val_t)
 ISVD_DEFINE(b, PrecB, float, isvd_b_val_t)
 ISVD_DEFINE_GPU_TWINS(s, isvd_s_val_t)
 ISVD_DEFINE_GPU_TWINS(d, isvd_d_val_t)
+ISVD_DEFINE_BACKWARD(s, PrecS, isvd_s_val_t)
+ISVD_DEFINE_BACKWARD(d, PrecD, isvd_d_val_t)
 
 }  // extern "C"

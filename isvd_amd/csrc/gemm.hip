@@ -344,6 +344,10 @@ void launch(GemmArgs<T> g, i64 batch) {
     launch_cfg<T, TA, TB, 64, 64, 16, 2, 2>(g, batch);
     return;
   }
+  if (g.n <= 128 && big_tile_cfg() == 0) {  // a 256-wide tile would be half empty
+    launch_cfg<T, TA, TB, 128, 128, 16, 2, 2>(g, batch);
+    return;
+  }
   switch (big_tile_cfg()) {
     case 1: launch_cfg<T, TA, TB, 128, 128, 16, 2, 2>(g, batch); break;
     case 2: launch_cfg<T, TA, TB, 256, 128, 16, 4, 2>(g, batch); break;

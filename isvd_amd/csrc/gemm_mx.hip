@@ -1,9 +1,11 @@
 // BF16 MFMA GEMM family for gfx950 with FP32 accumulation (v_mfma_f32_16x16x32_bf16).
 //
-// Workgroup tile 128 x 128, K step 64, 256 threads = 2 x 2 waves, each wave a 64 x 64 block
-// of 4 x 4 MFMA tiles.  Both operands are staged in LDS as [mn][k] rows of 64 BF16 values
-// (+8 padding), so every MFMA fragment (lane -> mn = lane%16, k = 8*(lane/16) .. +7) is one
-// 16-byte LDS read, conflict-free across each 16-lane group (row stride 36 dwords).
+// Workgroup tiles 256 x 256 (8 waves of 128 x 64) or, for the narrow l x n projection,
+// 128 x 128 (4 waves of 64 x 64); K step 64.  A 256 x 256 BF16 tile does 128 FLOP per byte it
+// reads through L2, twice the 128 x 128 tile's, which at the BF16 matrix peak would need all of
+// the L2 bandwidth.  Both operands are staged in LDS as [mn][k] rows of 64 BF16 values (+8
+// padding), so every MFMA fragment (lane -> mn = lane%16, k = 8*(lane/16) .. +7) is one 16-byte
+// LDS read, conflict-free across each 16-lane group (row stride 36 dwords).
 //  - K-contiguous sources are copied 16 bytes at a time.
 //  - MN-contiguous sources (the sketch's Omega^T and A^T, the Gram of the stacked bases) are
 //    loaded as 8 consecutive mn at 4 consecutive k per thread and transposed in registers into
@@ -23,7 +25,7 @@ typedef float f4v __attribute__((ext_vector_type(4)));
 
 namespace {
 
-constexpr int kBM = 128, kBN = 128, kBK = 64, kLK = kBK + 8;  // LDS row length (BF16 values)
+constexpr int kBM = 128, kBN = 128;
 
 struct MxArgs {
   i64 m, n, k;
@@ -57,12 +59,18 @@ __device__ __forceinline__ void split_bf16(float x, uint32_t (&h)[P]) {
   }
 }
 
-// One operand tile (128 mn x 64 k) in flight: global -> registers -> LDS (P parts).
-// T = uint16_t (BF16 bits, P = 1) or float (split into P parts).
-template <typename T, bool KC, int P>
+// One operand tile (R mn x KT k) in flight: global -> registers -> LDS (P parts).
+// T = uint16_t (BF16 bits, P = 1) or float (split into P parts).  NT threads; each thread
+// moves NL runs of 8 consecutive elements.
+template <typename T, bool KC, int P, int R, int KT, int NT>
 struct Tile {
   static constexpr int kW = (int)(8 * sizeof(T) / 16);  // 16-byte words per 8 elements
-  uint4 r[4][kW];
+  static constexpr int kLK = KT + 8;                     // LDS row length (BF16 values)
+  // K-contiguous: RUNS runs of 8 k; MN-contiguous: UNITS units of 8 mn x 4 k
+  static constexpr int RUNS = R * KT / 8, UNITS = R * KT / 32;
+  static constexpr int NL = KC ? (RUNS + NT - 1) / NT : 4 * ((UNITS + NT - 1) / NT);
+  static_assert(KC ? RUNS % NT == 0 : (UNITS % NT == 0 || UNITS < NT), "tile / thread mismatch");
+  uint4 r[NL][kW];
 
   __device__ __forceinline__ static float elem_f(const uint4 *w, int e) {
     const uint4 q = w[e >> 2];
@@ -95,27 +103,32 @@ struct Tile {
   __device__ __forceinline__ void load(const T *X, i64 ld, i64 mn0, i64 mnmax, i64 k0, i64 kend,
                                        bool vec, int tid) {
     if (KC) {
-      // unit u: row = c / 8, k = 8 * (c % 8), c = tid + 256 u
+      // run u: row c / (KT/8), k 8 (c % (KT/8)) .. +7, c = tid + NT u
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int c = tid + 256 * u;
-        const i64 mn = mn0 + (c >> 3), kk = k0 + 8 * (c & 7);
+      for (int u = 0; u < NL; ++u) {
+        const int c = tid + NT * u;
+        const i64 mn = mn0 + c / (KT / 8), kk = k0 + 8 * (c % (KT / 8));
         int valid = (mn < mnmax) ? (int)(kend - kk) : 0;
         if (valid < 0) valid = 0;
         if (valid > 8) valid = 8;
         load8(X + (valid > 0 ? kk + mn * ld : 0), valid, vec, r[u]);
       }
     } else {
-      // thread: mn = 8 (tid / 16) .. +7, k = 4 (tid % 16) + j (j < 4)
-      const i64 mn = mn0 + 8 * (tid >> 4);
-      int valid = (int)(mnmax - mn);
-      if (valid < 0) valid = 0;
-      if (valid > 8) valid = 8;
+      // unit c = tid + NT uu: mn 8 (c / (KT/4)) .. +7 at k 4 (c % (KT/4)) + j, j < 4
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const i64 kk = k0 + 4 * (tid & 15) + j;
-        const int v = (kk < kend) ? valid : 0;
-        load8(X + (v > 0 ? mn + kk * ld : 0), v, vec, r[j]);
+      for (int uu = 0; uu < NL / 4; ++uu) {
+        const int c = tid + NT * uu;
+        if (c >= UNITS) break;
+        const i64 mn = mn0 + 8 * (c / (KT / 4));
+        int valid = (int)(mnmax - mn);
+        if (valid < 0) valid = 0;
+        if (valid > 8) valid = 8;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const i64 kk = k0 + 4 * (c % (KT / 4)) + j;
+          const int v = (kk < kend) ? valid : 0;
+          load8(X + (v > 0 ? mn + kk * ld : 0), v, vec, r[4 * uu + j]);
+        }
       }
     }
   }
@@ -123,9 +136,9 @@ struct Tile {
   __device__ __forceinline__ void store(uint16_t *s, int tid) const {
     if (KC) {
 #pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const int c = tid + 256 * u;
-        uint16_t *dst = s + (c >> 3) * kLK + 8 * (c & 7);
+      for (int u = 0; u < NL; ++u) {
+        const int c = tid + NT * u;
+        uint16_t *dst = s + (c / (KT / 8)) * kLK + 8 * (c % (KT / 8));
         if (sizeof(T) == 2) {
           *reinterpret_cast<uint4 *>(dst) = r[u][0];
         } else {
@@ -139,30 +152,37 @@ struct Tile {
             w.y = h[2][p] | (h[3][p] << 16);
             w.z = h[4][p] | (h[5][p] << 16);
             w.w = h[6][p] | (h[7][p] << 16);
-            *reinterpret_cast<uint4 *>(dst + p * kBM * kLK) = w;
+            *reinterpret_cast<uint4 *>(dst + p * R * kLK) = w;
           }
         }
       }
     } else {
-      const int row0 = 8 * (tid >> 4), kq = 4 * (tid & 15);
+      // lanes with consecutive k groups write one contiguous LDS row span (conflict-free)
 #pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        uint16_t *dst = s + (row0 + e) * kLK + kq;
-        if (sizeof(T) == 2) {
-          uint2 w;
-          w.x = elem_h(r[0], e) | (elem_h(r[1], e) << 16);
-          w.y = elem_h(r[2], e) | (elem_h(r[3], e) << 16);
-          *reinterpret_cast<uint2 *>(dst) = w;
-        } else {
-          uint32_t h[4][P];
+      for (int uu = 0; uu < NL / 4; ++uu) {
+        const int c = tid + NT * uu;
+        if (c >= UNITS) break;
+        const int row0 = 8 * (c / (KT / 4)), kq = 4 * (c % (KT / 4));
+        const uint4(*q)[kW] = r + 4 * uu;
 #pragma unroll
-          for (int j = 0; j < 4; ++j) split_bf16<P>(elem_f(r[j], e), h[j]);
-#pragma unroll
-          for (int p = 0; p < P; ++p) {
+        for (int e = 0; e < 8; ++e) {
+          uint16_t *dst = s + (row0 + e) * kLK + kq;
+          if (sizeof(T) == 2) {
             uint2 w;
-            w.x = h[0][p] | (h[1][p] << 16);
-            w.y = h[2][p] | (h[3][p] << 16);
-            *reinterpret_cast<uint2 *>(dst + p * kBM * kLK) = w;
+            w.x = elem_h(q[0], e) | (elem_h(q[1], e) << 16);
+            w.y = elem_h(q[2], e) | (elem_h(q[3], e) << 16);
+            *reinterpret_cast<uint2 *>(dst) = w;
+          } else {
+            uint32_t h[4][P];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) split_bf16<P>(elem_f(q[j], e), h[j]);
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+              uint2 w;
+              w.x = h[0][p] | (h[1][p] << 16);
+              w.y = h[2][p] | (h[3][p] << 16);
+              *reinterpret_cast<uint2 *>(dst + p * R * kLK) = w;
+            }
           }
         }
       }
@@ -174,14 +194,28 @@ __device__ __forceinline__ bf16x8 frag(const uint16_t *s) {
   return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(s));
 }
 
-template <typename TA, bool AK, int PA, typename TB, bool BK, int PB>
-__global__ __launch_bounds__(256) void gemm_mx_kernel(MxArgs g) {
+// Workgroup tile BM x BN, K step KT, WGM x WGN waves (each a (BM/WGM) x (BN/WGN) block of
+// 16 x 16 MFMA tiles).
+template <int BM_, int BN_, int WGM_, int WGN_, int KT_>
+struct Cfg {
+  static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, KT = KT_;
+  static constexpr int NT = 64 * WGM * WGN;
+  static constexpr int FM = BM / WGM / 16, FN = BN / WGN / 16;
+};
+using Cfg256 = Cfg<256, 256, 2, 4, 64>;  // 8 waves of 128 x 64
+using CfgTall = Cfg<256, 128, 4, 2, 64>;  // 8 waves of 64 x 64
+using Cfg128 = Cfg<128, 128, 2, 2, 64>;  // 4 waves of 64 x 64
+
+template <typename TA, bool AK, int PA, typename TB, bool BK, int PB, class C>
+__global__ __launch_bounds__(C::NT) void gemm_mx_kernel(MxArgs g) {
   constexpr int PMAX = PA > PB ? PA : PB;
+  constexpr int kBM = C::BM, kBN = C::BN, kBK = C::KT, kLK = C::KT + 8;
+  constexpr int FM = C::FM, FN = C::FN;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   uint16_t *sA = reinterpret_cast<uint16_t *>(smem_raw);  // [PA][kBM][kLK]
   uint16_t *sB = sA + PA * kBM * kLK;                      // [PB][kBN][kLK]
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave & 1, wn = wave >> 1;
+  const int wm = wave % C::WGM, wn = wave / C::WGM;
 
   // ---- tile of this workgroup (XCD-contiguous; symmetric products walk the upper tiles)
   const i64 tiles_m = (g.m + kBM - 1) / kBM, tiles_n = (g.n + kBN - 1) / kBN;
@@ -222,19 +256,20 @@ __global__ __launch_bounds__(256) void gemm_mx_kernel(MxArgs g) {
   const i64 kend = (kbeg + g.kchunk < g.k) ? kbeg + g.kchunk : g.k;
   const int nk = (kend > kbeg) ? (int)((kend - kbeg + kBK - 1) / kBK) : 0;
 
-  Tile<TA, AK, PA> ta;
-  Tile<TB, BK, PB> tb;
-  f4v acc[4][4];
+  Tile<TA, AK, PA, kBM, kBK, C::NT> ta;
+  Tile<TB, BK, PB, kBN, kBK, C::NT> tb;
+  f4v acc[FM][FN];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < FM; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < FN; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 
   if (nk > 0) {
     ta.load(A, g.lda, m0, g.m, kbeg, kend, g.avec, tid);
     tb.load(B, g.ldb, n0, g.n, kbeg, kend, g.bvec, tid);
   }
   const int fr = lane & 15, fk = 8 * (lane >> 4);
+  const int wrow = wm * (kBM / C::WGM), wcol = wn * (kBN / C::WGN);
   for (int kt = 0; kt < nk; ++kt) {
     ta.store(sA, tid);
     tb.store(sB, tid);
@@ -246,20 +281,20 @@ __global__ __launch_bounds__(256) void gemm_mx_kernel(MxArgs g) {
     }
 #pragma unroll
     for (int s = 0; s < kBK / 32; ++s) {
-      bf16x8 bf[PB][4];
+      bf16x8 bf[PB][FN];
 #pragma unroll
       for (int p = 0; p < PB; ++p)
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj)
-          bf[p][tj] = frag(sB + p * kBN * kLK + (wn * 64 + tj * 16 + fr) * kLK + 32 * s + fk);
+        for (int tj = 0; tj < FN; ++tj)
+          bf[p][tj] = frag(sB + p * kBN * kLK + (wcol + tj * 16 + fr) * kLK + 32 * s + fk);
 #pragma unroll
-      for (int ti = 0; ti < 4; ++ti) {
+      for (int ti = 0; ti < FM; ++ti) {
         bf16x8 af[PA];
 #pragma unroll
         for (int p = 0; p < PA; ++p)
-          af[p] = frag(sA + p * kBM * kLK + (wm * 64 + ti * 16 + fr) * kLK + 32 * s + fk);
+          af[p] = frag(sA + p * kBM * kLK + (wrow + ti * 16 + fr) * kLK + 32 * s + fk);
 #pragma unroll
-        for (int tj = 0; tj < 4; ++tj)
+        for (int tj = 0; tj < FN; ++tj)
 #pragma unroll
           for (int ia = 0; ia < PA; ++ia)
 #pragma unroll
@@ -273,23 +308,23 @@ __global__ __launch_bounds__(256) void gemm_mx_kernel(MxArgs g) {
   }
 
   // ---- epilogue: D register q of a lane holds row 4*(lane/16) + q, column lane%16
-  float *C = g.C + batch_id * g.sC;
+  float *Cm = g.C + batch_id * g.sC;
   float *W = (g.ksplit > 1) ? g.W + ((i64)batch_id * g.ksplit + ks) * g.m * g.n : nullptr;
 #pragma unroll
-  for (int ti = 0; ti < 4; ++ti)
+  for (int ti = 0; ti < FM; ++ti)
 #pragma unroll
-    for (int tj = 0; tj < 4; ++tj)
+    for (int tj = 0; tj < FN; ++tj)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const i64 row = m0 + wm * 64 + ti * 16 + 4 * (lane >> 4) + q;
-        const i64 col = n0 + wn * 64 + tj * 16 + fr;
+        const i64 row = m0 + wrow + ti * 16 + 4 * (lane >> 4) + q;
+        const i64 col = n0 + wcol + tj * 16 + fr;
         if (row < g.m && col < g.n) {
           if (W) {
             W[row + col * g.m] = acc[ti][tj][q];
           } else {
             float v = g.alpha * acc[ti][tj][q];
-            if (g.beta != 0.f) v += g.beta * C[row + col * g.ldc];
-            C[row + col * g.ldc] = v;
+            if (g.beta != 0.f) v += g.beta * Cm[row + col * g.ldc];
+            Cm[row + col * g.ldc] = v;
           }
         }
       }
@@ -341,10 +376,11 @@ int grid_of(i64 n) {
 
 constexpr int kTargetBlocks = 1024;
 
-template <typename TA, bool AK, int PA, typename TB, bool BK, int PB>
+template <typename TA, bool AK, int PA, typename TB, bool BK, int PB, class C>
 void launch_mx(MxArgs g, i64 batch) {
-  auto kern = gemm_mx_kernel<TA, AK, PA, TB, BK, PB>;
-  const int lds = (PA + PB) * kBM * kLK * 2;
+  constexpr int kBM = C::BM, kBN = C::BN, kBK = C::KT;
+  auto kern = gemm_mx_kernel<TA, AK, PA, TB, BK, PB, C>;
+  const int lds = (PA * kBM + PB * kBN) * (kBK + 8) * 2;
   static std::once_flag once;
   std::call_once(once, [&] {
     HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -359,9 +395,11 @@ void launch_mx(MxArgs g, i64 batch) {
       tiles += c < tiles_m ? c : tiles_m;
     }
   }
+  // split K until the launch has ~4 workgroups of 256 threads per CU
+  const i64 target = kTargetBlocks * 256 / C::NT;
   int ksplit = 1;
-  if (tiles * batch < kTargetBlocks && g.k >= 8 * kBK) {
-    i64 want = (kTargetBlocks + tiles * batch - 1) / (tiles * batch);
+  if (tiles * batch < target && g.k >= 8 * kBK) {
+    i64 want = (target + tiles * batch - 1) / (tiles * batch);
     const i64 maxs = g.k / (4 * kBK);
     if (want > maxs) want = maxs;
     if (want > 64) want = 64;
@@ -382,7 +420,7 @@ void launch_mx(MxArgs g, i64 batch) {
   }
   ISVD_CHECK(batch * ksplit <= 65535, "grid.y too large (batch %lld x ksplit %d)", batch, ksplit);
   ISVD_CHECK(tiles <= 0x7fffffff, "too many tiles (%lld)", tiles);
-  hipLaunchKernelGGL(kern, dim3((unsigned)tiles, (unsigned)(batch * ksplit)), dim3(256), lds,
+  hipLaunchKernelGGL(kern, dim3((unsigned)tiles, (unsigned)(batch * ksplit)), dim3(C::NT), lds,
                      ctx().stream, g);
   LAUNCH_CHECK();
   if (ksplit > 1) {
@@ -391,6 +429,32 @@ void launch_mx(MxArgs g, i64 batch) {
                        g.ldc, g.sC);
     LAUNCH_CHECK();
   }
+}
+
+// dst (cols x rows, ldd) := src (rows x cols, lds)^T for 16-bit elements, 64 x 64 LDS tiles
+__global__ __launch_bounds__(256) void transpose16_kernel(i64 rows, i64 cols, const uint16_t *src,
+                                                          i64 lds, uint16_t *dst, i64 ldd) {
+  __shared__ uint16_t t[64][66];
+  const i64 i0 = blockIdx.x * 64ll, j0 = blockIdx.y * 64ll;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int jj = ty + 4 * r;
+    const i64 i = i0 + tx, j = j0 + jj;
+    t[jj][tx] = (i < rows && j < cols) ? src[i + j * lds] : 0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int ii = ty + 4 * r;
+    const i64 j = j0 + tx, i = i0 + ii;
+    if (i < rows && j < cols) dst[j + i * ldd] = t[tx][ii];
+  }
+}
+
+int env_int(const char *name, int dflt) {
+  const char *e = std::getenv(name);
+  return e ? std::atoi(e) : dflt;
 }
 
 bool aligned16(const void *p, i64 ld, i64 stride, i64 esize) {
@@ -416,19 +480,38 @@ void gemm_mx(const MxOperand &a, const MxOperand &b, i64 m, i64 n, i64 k, float 
   g.avec = aligned16(a.p, a.ld, a.stride, a.f32 ? 4 : 2);
   g.bvec = aligned16(b.p, b.ld, b.stride, b.f32 ? 4 : 2);
   using h16 = uint16_t;
-  // the operand combinations of the mixed pipeline
-  if (!a.f32 && !a.kc && !b.f32 && !b.kc)
-    launch_mx<h16, false, 1, h16, false, 1>(g, batch);  // sketch, A column-major
-  else if (!a.f32 && !a.kc && !b.f32 && b.kc)
-    launch_mx<h16, false, 1, h16, true, 1>(g, batch);   // sketch, A row-major
-  else if (a.f32 && !a.kc && a.parts == 2 && b.f32 && !b.kc && b.parts == 2)
-    launch_mx<float, false, 2, float, false, 2>(g, batch);  // Gram of the stacked bases
-  else if (a.f32 && !a.kc && a.parts == 3 && !b.f32 && b.kc)
-    launch_mx<float, false, 3, h16, true, 1>(g, batch);   // Qbar^T A, A column-major
-  else if (a.f32 && !a.kc && a.parts == 3 && !b.f32 && !b.kc)
-    launch_mx<float, false, 3, h16, false, 1>(g, batch);  // Qbar^T A, A row-major
-  else
+  // 256 x 256 tiles (8 waves) when both output dimensions can use them; the l x n projection
+  // (m = l <= 128) runs 128 x 128 tiles
+  const int tile = env_int("ISVD_MX_TILE", 256);
+  const bool wide = (m >= 256 && n >= 256);
+  if (!a.f32 && !b.f32) {  // sketch: BF16 x BF16
+    const int cfg = !wide ? 0 : (tile == 256 ? 2 : tile == 128 ? 0 : 1);
+    const int code = (a.kc ? 2 : 0) + (b.kc ? 1 : 0) + 4 * cfg;
+    switch (code) {
+      case 0: launch_mx<h16, false, 1, h16, false, 1, Cfg128>(g, batch); break;
+      case 1: launch_mx<h16, false, 1, h16, true, 1, Cfg128>(g, batch); break;
+      case 2: launch_mx<h16, true, 1, h16, false, 1, Cfg128>(g, batch); break;
+      case 3: launch_mx<h16, true, 1, h16, true, 1, Cfg128>(g, batch); break;
+      case 4: launch_mx<h16, false, 1, h16, false, 1, CfgTall>(g, batch); break;
+      case 5: launch_mx<h16, false, 1, h16, true, 1, CfgTall>(g, batch); break;
+      case 6: launch_mx<h16, true, 1, h16, false, 1, CfgTall>(g, batch); break;
+      case 7: launch_mx<h16, true, 1, h16, true, 1, CfgTall>(g, batch); break;
+      case 8: launch_mx<h16, false, 1, h16, false, 1, Cfg256>(g, batch); break;
+      case 9: launch_mx<h16, false, 1, h16, true, 1, Cfg256>(g, batch); break;
+      case 10: launch_mx<h16, true, 1, h16, false, 1, Cfg256>(g, batch); break;
+      default: launch_mx<h16, true, 1, h16, true, 1, Cfg256>(g, batch); break;
+    }
+  } else if (a.f32 && !a.kc && a.parts == 2 && b.f32 && !b.kc && b.parts == 2) {
+    // Gram of the stacked bases
+    if (wide) launch_mx<float, false, 2, float, false, 2, CfgTall>(g, batch);
+    else launch_mx<float, false, 2, float, false, 2, Cfg128>(g, batch);
+  } else if (a.f32 && !a.kc && a.parts == 3 && !b.f32 && b.kc) {
+    launch_mx<float, false, 3, h16, true, 1, Cfg128>(g, batch);   // Qbar^T A, A column-major
+  } else if (a.f32 && !a.kc && a.parts == 3 && !b.f32 && !b.kc) {
+    launch_mx<float, false, 3, h16, false, 1, Cfg128>(g, batch);  // Qbar^T A, A row-major
+  } else {
     fail("gemm_mx: operand combination not built");
+  }
   if (upper) {
     hipLaunchKernelGGL(mx_mirror_kernel, dim3(grid_of(n * n)), dim3(256), 0, ctx().stream, n, C,
                        ldc);
@@ -441,6 +524,13 @@ void convert(i64 rows, i64 cols, const Tin *src, i64 lds, Tout *dst, i64 ldd) {
   if (rows <= 0 || cols <= 0) return;
   hipLaunchKernelGGL((convert_kernel<Tout, Tin>), dim3(grid_of(rows * cols)), dim3(256), 0,
                      ctx().stream, rows, cols, src, lds, dst, ldd);
+  LAUNCH_CHECK();
+}
+
+void transpose16(i64 rows, i64 cols, const uint16_t *src, i64 lds, uint16_t *dst, i64 ldd) {
+  if (rows <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL(transpose16_kernel, dim3((unsigned)((rows + 63) / 64), (unsigned)((cols + 63) / 64)),
+                     dim3(256), 0, ctx().stream, rows, cols, src, lds, dst, ldd);
   LAUNCH_CHECK();
 }
 

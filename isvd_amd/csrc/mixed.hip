@@ -2,6 +2,8 @@
 // cited at each function).
 #include "mixed.hpp"
 
+#include <cstdlib>
+
 #include "la.hpp"
 #include "mx.hpp"
 
@@ -46,6 +48,19 @@ void stage_sketch_b(const Dims &d, char ordera, const uint16_t *a, i64 lda,
   om.p = omegat; om.ld = ldo; om.f32 = false; om.kc = false;
   // op(A)^T(kk, j) = A(j, kk): column-major A has j contiguous, row-major A has kk contiguous
   am.p = a; am.ld = lda; am.f32 = false; am.kc = (ordera == 'R');
+  const char *e = std::getenv("ISVD_MX_SKETCH_T");
+  const int tmode = e ? std::atoi(e) : 0;
+  DBuf<uint16_t> omt, at;
+  if (tmode >= 1) {  // Omega^T to K-contiguous: Omega (n x Nl, ld Nl) as Nl x n col-major
+    omt.alloc(d.n * d.Nl);
+    transpose16(d.Nl, d.n, omegat, ldo, omt.get(), d.n);
+    om.p = omt.get(); om.ld = d.n; om.kc = true;
+  }
+  if (tmode >= 2 && ordera == 'C') {  // A to row-major
+    at.alloc(d.m * d.n);
+    transpose16(d.m, d.n, a, lda, at.get(), d.n);
+    am.p = at.get(); am.ld = d.n; am.kc = true;
+  }
   gemm_mx(om, am, d.Nl, d.m, d.n, 1.f, 0.f, yst, ldyst);
 }
 

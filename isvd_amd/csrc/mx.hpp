@@ -32,6 +32,9 @@ void gemm_mx(const MxOperand &a, const MxOperand &b, i64 m, i64 n, i64 k, float 
 template <typename Tout, typename Tin>
 void convert(i64 rows, i64 cols, const Tin *src, i64 lds, Tout *dst, i64 ldd);
 
+// dst (cols x rows, ldd) := src (rows x cols, lds)^T, 16-bit elements
+void transpose16(i64 rows, i64 cols, const uint16_t *src, i64 lds, uint16_t *dst, i64 ldd);
+
 // BF16 bits (round to nearest even) of an FP32 array
 void f32_to_bf16(i64 count, const float *src, uint16_t *dst);
 

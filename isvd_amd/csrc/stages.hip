@@ -247,79 +247,126 @@ void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst) {
 //   Bg = Qs^T (Qs B / N) = M B / N with M = Qs^T Qs (Nl x Nl, formed once),
 //   X_+ = X_c F + (B_c / N) C^-1   (the image of Q_+ = Q_c F + G_c C^-1),
 // so each iteration touches only Nl x l and l x l matrices; Qbar = Qs X is formed at the end.
+// Newton-Schulz step counts for kn_matfun from rho = ||4 Z||_F (>= the spectral radius of 4 Z).
+// The coupled iteration on A = I - R has residual R_k = I - W_k Y_k with
+// ||R_{k+1}|| <= ||R_k||^2 (3 + ||R_k||) / 4; stage 1 starts from ||R_0|| <= rho, stage 2 from
+// ||I - (I + P)/2|| = max |1 - sqrt(1 - x)| / 2 over |x| <= rho <= rho / (2 (1 + sqrt(1 - rho))).
+// Each stage runs until the bound falls below the unit roundoff.
+template <typename T>
+void ns_counts(double rho, int *k1, int *k2) {
+  const double eps = (sizeof(T) == 8) ? 0x1p-53 : 0x1p-24;
+  auto steps = [&](double r) {
+    int k = 0;
+    while (r > eps && k < 12) {
+      r = r * r * (3.0 + r) / 4.0;
+      ++k;
+    }
+    return k < 1 ? 1 : k;
+  };
+  *k1 = steps(rho);
+  *k2 = steps(rho / (2.0 * (1.0 + std::sqrt(1.0 - rho))));
+}
+
+// [W1 | W2] (2l x l each, ld 2l) := [F; C^-1 / N] and [F; C^-1]
+template <typename T>
+__global__ void kn_assemble_kernel(int l, const T *F, const T *Ci, T ninv, T *W) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < l * l; e += gridDim.x * blockDim.x) {
+    const int i = e % l, j = e / l;
+    const T f = F[e], ci = Ci[e];
+    T *w1 = W + (i64)j * 2 * l, *w2 = W + (i64)2 * l * l + (i64)j * 2 * l;
+    w1[i] = f;
+    w1[l + i] = ninv * ci;
+    w2[i] = f;
+    w2[l + i] = ci;
+  }
+}
+
+// The KN iteration in sketch coordinates.  Per iteration (one host synchronisation):
+//   [X | B | Bg] (Nl x 3l) holds X_c, B_c = Qs^T Q_c and Bg_c = M B_c / N;
+//   [D | Dg] = B_c^T [B_c | Bg_c] / N (one product), Z = Dg - D^T D, rho = ||4 Z||_F;
+//   C, C^-1 from Z (Newton-Schulz with step counts chosen from rho, or the eigendecomposition
+//   when rho >= 0.6);  F = C - D C^-1;
+//   [X_+ | B_+] = [X_c B_c] [F; C^-1/N] and [B_c Bg_c] [F; C^-1] (one batched product);
+//   then Bg, D, Dg, Z and rho of the next iterate are formed before the host reads err and
+//   the next rho together (the work after the last iteration is discarded).
 template <typename T>
 void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out, i64 *iter_out,
                          double *err_out) {
   const i64 l = d.l, Nl = d.Nl;
   const T ninv = T(1) / T(d.N);
-  DBuf<T> xc(Nl * l), xp(Nl * l), bc(Nl * l), bp(Nl * l), bgc(Nl * l);
-  DBuf<T> dc(l * l), z(l * l), w(l), zv(l * l), zs(l * l), zi(l * l), c(l * l), cinv(l * l),
-      err(1);
+  DBuf<T> xbuf(2 * Nl * 3 * l), dz(2 * l * l), w(l), zv(l * l), zs(l * l), zi(l * l), c(l * l),
+      cinv(l * l), wmat(4 * l * l), scal(2), work(6 * l * l);
+  T *xb[2] = {xbuf.get(), xbuf.get() + Nl * 3 * l};
+  T *D = dz.get(), *Z = dz.get() + l * l;  // [D | Dg]; Dg becomes Z in place
+  // Bg, [D | Dg], Z and rho = ||Z||_F (scal[1]) of the iterate held in xb[q]
+  auto prep = [&](int q) {
+    T *X = xb[q], *B = X + Nl * l, *Bg = X + 2 * Nl * l;
+    (void)X;
+    // Bg := M B / N  (:174-178)
+    gemm<T>('N', 'N', Nl, l, Nl, ninv, M, Nl, B, Nl, T(0), Bg, Nl);
+    // D := B^T B / N (:181), Dg := B^T Bg / N (:184-186)
+    gemm<T>('T', 'N', l, 2 * l, Nl, ninv, B, Nl, B, Nl, T(0), D, l);
+    // Z := Dg - D^T D (:187-190)
+    small_gemm<T>('T', 'N', l, l, l, T(-1), D, l, D, l, T(1), Z, l);
+    frob_shift<T>(l, Z, T(0), scal.get() + 1);
+  };
   // X_c := [I; 0] (Qc := Q_0, :156);  B_c := Qs^T Q_0 = M[:, 0:l]  (:159-160)
-  fill<T>(Nl, l, T(0), xc.get(), Nl, T(1));
-  omatcopy<T>('N', Nl, l, T(1), M, Nl, bc.get(), Nl);
+  fill<T>(Nl, l, T(0), xb[0], Nl, T(1));
+  omatcopy<T>('N', Nl, l, T(1), M, Nl, xb[0] + Nl * l, Nl);
+  prep(0);
+  T h[2];
+  HIP_CHECK(hipMemcpyAsync(&h[1], scal.get() + 1, sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
+  sync();
+  int cur = 0;
   i64 it;
   double error = -INFINITY;
-  DBuf<T> work(6 * l * l), rho(1);
   for (it = 1;; ++it) {
-    // Bgc := M Bc / N  (:174-178)
-    gemm<T>('N', 'N', Nl, l, Nl, ninv, M, Nl, bc.get(), Nl, T(0), bgc.get(), Nl);
-    // Dc := Bc^T Bc / N (:181); Z := Bc^T Bgc / N - Dc^T Dc (:184-190)
-    gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bc.get(), Nl, T(0), dc.get(), l);
-    gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bgc.get(), Nl, T(0), z.get(), l);
-    small_gemm<T>('T', 'N', l, l, l, T(-1), dc.get(), l, dc.get(), l, T(1), z.get(), l);
-    // C and inv(C) (:193-212): C = (I/2 + (I/4 - Z)^{1/2})^{1/2} = Z diag(s) Z^T with
-    // s = sqrt(1/2 + sqrt(1/4 - eig(Z))).  Newton-Schulz square roots while ||4 Z|| < 0.6
-    // (checked below from ||Z||_F), the eigendecomposition otherwise.
-    bool use_eig = false;
-    for (int attempt = 0; attempt < 2; ++attempt) {
-      if (!use_eig) {
-        frob_shift<T>(l, z.get(), T(0), rho.get());
-        kn_matfun<T>(l, z.get(), c.get(), cinv.get(), work.get(), 7, 6);
-        frob_shift<T>(l, c.get(), T(1), err.get());  // ||I - C||_F = ||s - 1||_2
-      } else {
-        syev_small<T>(l, z.get(), l, w.get(), zv.get(), l);
-        hipLaunchKernelGGL(kn_scalars_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
-                           (const T *)w.get(), (const T *)zv.get(), zs.get(), zi.get(),
-                           err.get());
-        LAUNCH_CHECK();
-        small_gemm<T>('N', 'T', l, l, l, T(1), zs.get(), l, zs.get(), l, T(0), c.get(), l);
-        small_gemm<T>('N', 'T', l, l, l, T(1), zi.get(), l, zi.get(), l, T(0), cinv.get(), l);
-        hipLaunchKernelGGL(sym_from_upper_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
-                           ctx().stream, (int)l, cinv.get(), l);
-        LAUNCH_CHECK();
-      }
-      // Fc [in C] := C - Dc inv(C)  (:218)
-      small_gemm<T>('N', 'N', l, l, l, T(-1), dc.get(), l, cinv.get(), l, T(1), c.get(), l);
-      // X_+ := X_c Fc + (Bc / N) inv(C)  (image of :221-222)
-      gemm<T>('N', 'N', Nl, l, l, T(1), xc.get(), Nl, c.get(), l, T(0), xp.get(), Nl);
-      gemm<T>('N', 'N', Nl, l, l, ninv, bc.get(), Nl, cinv.get(), l, T(1), xp.get(), Nl);
-      // B_+ := Bc Fc + Bgc inv(C)  (:225-226)
-      gemm<T>('N', 'N', Nl, l, l, T(1), bc.get(), Nl, c.get(), l, T(0), bp.get(), Nl);
-      gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, cinv.get(), l, T(1), bp.get(), Nl);
-      T h[2];
-      HIP_CHECK(hipMemcpyAsync(&h[0], err.get(), sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
-      if (!use_eig)
-        HIP_CHECK(hipMemcpyAsync(&h[1], rho.get(), sizeof(T), hipMemcpyDeviceToHost,
-                                 ctx().stream));
-      sync();
-      error = (double)h[0];
-      if (use_eig || 4.0 * (double)h[1] < 0.6) break;
-      use_eig = true;  // Newton-Schulz not guaranteed to converge: redo with eig(Z)
+    const double rho = 4.0 * (double)h[1];
+    // C and inv(C) (:193-212): C = (I/2 + (I/4 - Z)^{1/2})^{1/2} = V diag(s) V^T with
+    // s = sqrt(1/2 + sqrt(1/4 - eig(Z)))
+    const bool use_eig = !(rho < 0.6);
+    if (!use_eig) {
+      int k1, k2;
+      ns_counts<T>(rho, &k1, &k2);
+      kn_matfun<T>(l, Z, c.get(), cinv.get(), work.get(), k1, k2);
+      frob_shift<T>(l, c.get(), T(1), scal.get());  // ||I - C||_F = ||s - 1||_2
+    } else {
+      syev_small<T>(l, Z, l, w.get(), zv.get(), l);
+      hipLaunchKernelGGL(kn_scalars_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
+                         (const T *)w.get(), (const T *)zv.get(), zs.get(), zi.get(), scal.get());
+      LAUNCH_CHECK();
+      small_gemm<T>('N', 'T', l, l, l, T(1), zs.get(), l, zs.get(), l, T(0), c.get(), l);
+      small_gemm<T>('N', 'T', l, l, l, T(1), zi.get(), l, zi.get(), l, T(0), cinv.get(), l);
+      hipLaunchKernelGGL(sym_from_upper_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
+                         ctx().stream, (int)l, cinv.get(), l);
+      LAUNCH_CHECK();
     }
+    // F [in C] := C - D inv(C)  (:218)
+    small_gemm<T>('N', 'N', l, l, l, T(-1), D, l, cinv.get(), l, T(1), c.get(), l);
+    hipLaunchKernelGGL(kn_assemble_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
+                       (int)l, (const T *)c.get(), (const T *)cinv.get(), ninv, wmat.get());
+    LAUNCH_CHECK();
+    // X_+ := X_c F + (B_c / N) inv(C) (image of :221-222);  B_+ := B_c F + Bg_c inv(C) (:225-226)
+    const int nxt = cur ^ 1;
+    gemm<T>('N', 'N', Nl, l, 2 * l, T(1), xb[cur], Nl, wmat.get(), 2 * l, T(0), xb[nxt], Nl, 2,
+            Nl * l, 2 * l * l, Nl * l);
+    // the next iterate's Bg, D, Z (needed unless this iteration is the last)
+    prep(nxt);
+    HIP_CHECK(hipMemcpyAsync(h, scal.get(), 2 * sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
+    sync();
+    error = (double)h[0];
     if (trace_on())
-      std::fprintf(stderr, "isvd KN iter %lld err %.17g (%s)\n", it, error,
+      std::fprintf(stderr, "isvd KN iter %lld err %.17g rho %.3g (%s)\n", it, error, rho,
                    use_eig ? "eig" : "newton-schulz");
+    cur = nxt;
     // convergence (:230-241)
     if (error <= args.tol) break;
     if (it == args.maxit) {
       ++it;
       break;
     }
-    std::swap(xc.p, xp.p);
-    std::swap(bc.p, bp.p);
   }
-  omatcopy<T>('N', Nl, l, T(1), xp.get(), Nl, x_out, Nl);
+  omatcopy<T>('N', Nl, l, T(1), xb[cur], Nl, x_out, Nl);
   *iter_out = it;
   *err_out = error;
 }

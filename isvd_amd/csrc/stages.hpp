@@ -74,4 +74,12 @@ template <typename T>
 void post_tsqr_z(const Dims &d, const T *qt, i64 ldqt, T *zt, T *s, T *ut, i64 ldut, T *vt,
                  i64 ldvt, bool want_u, bool want_v);
 
+// dL/dA of the rank-k truncated SVD from dL/dS (k), dL/dU^T (k x m, ldgut) and dL/dV^T
+// (k x n, ldgvt); U, S, V are recomputed from Qbar^T by the Gramian (tsqr = false) or TSQR
+// postprocessing.  dA has the storage order of A (ordera).
+template <typename T>
+void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 lda,
+                         const T *qt, i64 ldqt, const T *gs, const T *gut, i64 ldgut,
+                         const T *gvt, i64 ldgvt, T *da, i64 ldda);
+
 }  // namespace isvd

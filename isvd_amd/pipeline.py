@@ -144,3 +144,28 @@ def isvd(param, a, omega=None, seed=0, algs=("GP", "GR", "KN", "GR"), ordera="R"
     it, err = pack.ret(2)[:2]
     return {"s": s, "u": u, "v": v, "iter": int(it), "err": float(err),
             "time": list(pack.time)}
+
+
+def isvd_backward(param, a, qbar, gs=None, gu=None, gv=None, alg_p="TS", ordera="R", dista="R"):
+    """dL/dA of the rank-k result (isvd_?IsvdBackward).  ``qbar`` is the integration output
+    (m x l), ``gs`` (k), ``gu`` (m x k) and ``gv`` (n x k) the gradients of the outputs (None for
+    zero).  Returns dL/dA with the shape and storage of ``a``."""
+    lib = capi.load()
+    x, dt = _family(a)
+    if x == "b":
+        raise ValueError("isvd_backward: the mixed-precision family has no adjoint")
+    k = param.rank
+    da = _empty_like_kind(a, tuple(a.shape), dt)
+
+    def cont(t):
+        if t is None:
+            return None
+        return t.contiguous() if _is_torch(t) else np.ascontiguousarray(t)
+
+    gs, gu, gv = cont(gs), cont(gu), cont(gv)
+    getattr(lib, f"isvd_{x}IsvdBackward")(
+        param, alg_p.encode(), capi._chr(dista), capi._chr(ordera), capi.addr(a),
+        _a_ld(a, ordera), capi.addr(qbar), qbar.shape[1], capi.addr(gs), capi.addr(gu), k,
+        capi.addr(gv), k, capi.addr(da), _a_ld(a, ordera))
+    _check("isvd_backward")
+    return da
