@@ -111,8 +111,8 @@ int isvd_GetErrorCount(void);        /* failures since load */
 /* =========================================================================================
  * Driver: replaces isvd_?Isvd (src/include/c/isvd/core/@x@_driver.h:21-29,
  * src/lib/libisvd/core/driver/@x@_isvd.c:106-282).  Selectors (case-insensitive):
- *   alg_s "GP" | "GP_gpu";  alg_o "TS" | "GR";  alg_i "KN" | "WY";
- *   alg_p "TS" | "GR" | "TS_gpu" | "GR_gpu".   All run on the GPU.
+ *   alg_s "GP" | "GP_gpu";  alg_o "TS" | "GR";  alg_i "KN" | "WY" | "HR";
+ *   alg_p "TS" | "GR" | "SY" | "TS_gpu" | "GR_gpu" | "SY_gpu".   All run on the GPU.
  * time[4] receives per-stage seconds (GPU event time, stream-ordered).
  * ========================================================================================= */
 #define ISVD_DECLARE_DRIVER(x, T)                                                              \
@@ -182,6 +182,26 @@ ISVD_DECLARE_DRIVER_OMEGA(d, isvd_d_val_t)
   void isvd_##x##IntegrateWenYin(const isvd_Param param, const T *argv, const isvd_int_t argc, \
                                  T *retv, const isvd_int_t retc, const T *yst,                \
                                  const isvd_int_t ldyst, T *qt, const isvd_int_t ldqt);       \
+  /* @x@_stage.h:49-53 / @x@_integrate_hierarchical_reduction.c:36-141 (yst is destroyed) */   \
+  void isvd_##x##IntegrateHierarchicalReduction(const isvd_Param param, const T *argv,         \
+                                                const isvd_int_t argc, T *retv,                \
+                                                const isvd_int_t retc, T *yst,                 \
+                                                const isvd_int_t ldyst, T *qt,                 \
+                                                const isvd_int_t ldqt);                        \
+  /* @x@_stage.h:71-77 / @x@_postprocess_symmetric.c:64-193 (m == n; s ascending; V = U) */    \
+  void isvd_##x##PostprocessSymmetric(                                                         \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root);                                                               \
+  /* gpu/@x@_stage.h:74-80 */                                                                 \
+  void isvd_##x##PostprocessSymmetric_gpu(                                                     \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root);                                                               \
   /* @x@_stage.h:55-62 / @x@_postprocess_tall_skinny_qr.c:67-309 */                           \
   void isvd_##x##PostprocessTallSkinnyQr(                                                      \
       const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \

@@ -90,6 +90,11 @@ def load(path=None):
         _sig(getattr(lib, f"isvd_{x}IsvdOmega"),
              drv + [VP, I, VP, VP, I, VP, I, mpi_int_t, mpi_int_t])
     for x in "sd":
+        _sig(getattr(lib, f"isvd_{x}IntegrateHierarchicalReduction"),
+             [Param, VP, I, VP, I, VP, I, VP, I])
+        post = [Param, VP, I, VP, I, CH, CH, VP, I, VP, I, VP, VP, I, VP, I, mpi_int_t, mpi_int_t]
+        _sig(getattr(lib, f"isvd_{x}PostprocessSymmetric"), post)
+        _sig(getattr(lib, f"isvd_{x}PostprocessSymmetric_gpu"), post)
         _sig(getattr(lib, f"isvd_{x}IsvdBackward"),
              [Param, ctypes.c_char_p, CH, CH, VP, I, VP, I, VP, VP, I, VP, I, VP, I])
     _lib = lib

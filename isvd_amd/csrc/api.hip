@@ -328,13 +328,33 @@ static void wy_impl(const isvd_Param &param, const typename P::T *argv, i64 argc
   if (retc > 1) retv[1] = (T)mu;
 }
 
+// @x@_integrate_hierarchical_reduction.c:36-141 (yst is destroyed, as there)
+template <class P>
+static void hr_impl(const isvd_Param &param, i64 argc, typename P::T *yst, i64 ldyst,
+                    typename P::T *qt, i64 ldqt) {
+  using T = typename P::T;
+  if (argc < 0) return;
+  check_param(param);
+  const Dims d = dims_of(param);
+  ISVD_CHECK(ldyst >= d.Nl, "ldyst (%lld) must be >= Nl", ldyst);
+  ISVD_CHECK(ldqt >= d.l, "ldqt (%lld) must be >= l", ldqt);
+  ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  ensure_init();
+  Arg<T> Y(yst, span(d.Nl, d.m, ldyst), true, false);
+  Arg<T> Q(qt, span(d.l, d.m, ldqt), false, true);
+  stage_integrate_hr<T>(d, Y.dev, ldyst, Q.dev, ldqt);
+  Q.finish();
+  sync();
+}
+
 template <class P>
 static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera,
                       const typename P::TA *a, i64 lda, const typename P::T *qt, i64 ldqt,
                       typename P::T *s, typename P::T *ut, i64 ldut, typename P::T *vt,
-                      i64 ldvt, int ut_root, int vt_root, bool tsqr) {
+                      i64 ldvt, int ut_root, int vt_root, char kind) {
   using T = typename P::T;
   using TA = typename P::TA;
+  const bool tsqr = (kind == 'T');
   if (argc < 0) return;
   const char dista_ = arg2char("DISTA", dista, "CR");
   const char ordera_ = arg2char("ORDERA", ordera, "CR");
@@ -353,11 +373,32 @@ static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera
   const bool want_u = ut_root >= -1, want_v = vt_root >= -1;
   if (want_u) ISVD_CHECK(ut != nullptr, "ut is NULL");
   if (want_v) ISVD_CHECK(vt != nullptr, "vt is NULL");
+  if (kind == 'S') {
+    ISVD_CHECK(!P::mixed, "symmetric postprocessing has no mixed-precision form");
+    ISVD_CHECK(d.m == d.n, "symmetric postprocessing needs m == n (m = %lld, n = %lld)", d.m,
+               d.n);
+  }
   ensure_init();
   const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
   Arg<TA> A(a, a_span, true, false);
   Arg<T> Q(qt, span(d.l, d.m, ldqt), true, false);
   Arg<T> S(s, d.l, false, true);
+  if constexpr (!P::mixed) {
+    if (kind == 'S') {  // @x@_postprocess_symmetric.c:110-114, :163-171: U goes to vt if ut is skipped
+      T *u_ = want_u ? ut : vt;
+      const i64 ldu_ = want_u ? ldut : ldvt;
+      const bool want = want_u || want_v;
+      Arg<T> U(want ? u_ : nullptr, span(d.k, d.m, ldu_), false, true);
+      stage_post_symmetric<T>(d, ordera_, A.dev, lda, Q.dev, ldqt, S.dev, U.dev, ldu_, want);
+      Arg<T> V(want_u && want_v ? vt : nullptr, span(d.k, d.n, ldvt), false, true);
+      if (want_u && want_v) omatcopy<T>('N', d.k, d.m, T(1), U.dev, ldut, V.dev, ldvt);
+      S.finish();
+      U.finish();
+      V.finish();
+      sync();
+      return;
+    }
+  }
   Arg<T> U(want_u ? ut : nullptr, span(d.k, d.m, ldut), false, true);
   Arg<T> V(want_v ? vt : nullptr, span(d.k, d.n, ldvt), false, true);
   if constexpr (P::mixed)
@@ -380,16 +421,22 @@ static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera
 // ---------------------------------------------------------------------------------------------
 static const char *const kOptsS[] = {"GP", "GP_gpu"};
 static const char *const kOptsO[] = {"TS", "GR"};
-static const char *const kOptsI[] = {"KN", "WY"};
-static const char *const kOptsP[] = {"TS", "GR", "TS_gpu", "GR_gpu"};
+static const char *const kOptsI[] = {"KN", "WY", "HR"};
+static const char *const kOptsP[] = {"TS", "GR", "SY", "TS_gpu", "GR_gpu", "SY_gpu"};
 
 static const char *alg_name(char kind, const char *alg, const char *prec) {
   static thread_local std::string s;
   std::string name;
   if (kind == 'S') name = "Gaussian Projection Sketching";
   else if (kind == 'O') name = (alg[0] == 'T') ? "Tall-Skinny QR Orthogonalization" : "Gramian Orthogonalization";
-  else if (kind == 'I') name = (alg[0] == 'K') ? "Kolmogorov-Nagumo Integration" : "Wen-Yin Integration";
-  else name = (alg[0] == 'T') ? "Tall-Skinny QR Postprocessing" : "Gramian Postprocessing";
+  else if (kind == 'I')
+    name = (alg[0] == 'K')   ? "Kolmogorov-Nagumo Integration"
+           : (alg[0] == 'W') ? "Wen-Yin Integration"
+                             : "Hierarchical Reduction Integration";
+  else
+    name = (alg[0] == 'T')   ? "Tall-Skinny QR Postprocessing"
+           : (alg[0] == 'G') ? "Gramian Postprocessing"
+                             : "Symmetric Postprocessing";
   s = std::string(prec) + " GPU (MI355X) " + name;
   return s.c_str();
 }
@@ -407,8 +454,8 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   using TA = typename P::TA;
   const char *as = arg2str("ALG_S", alg_s, kOptsS, 2);
   const char *ao = arg2str("ALG_O", alg_o, kOptsO, 2);
-  const char *ai = arg2str("ALG_I", alg_i, kOptsI, 2);
-  const char *ap = arg2str("ALG_P", alg_p, kOptsP, 4);
+  const char *ai = arg2str("ALG_I", alg_i, kOptsI, 3);
+  const char *ap = arg2str("ALG_P", alg_p, kOptsP, 6);
   if (!as || !ao || !ai || !ap) return;
   const T *argv_i = argv ? argv[2] : nullptr;
   const i64 argc_s = argv ? argc[0] : 0, argc_o = argv ? argc[1] : 0, argc_i = argv ? argc[2] : 0,
@@ -417,12 +464,12 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   const i64 retc_i = retv ? retc[2] : 0;
   if (argc_i > 0) ISVD_CHECK(argv_i != nullptr, "argv[2] is NULL");
   if (retc_i > 0) ISVD_CHECK(retv_i != nullptr, "retv[2] is NULL");
-  const bool kn = (ai[0] == 'K');
+  const bool kn = (ai[0] == 'K'), hr = (ai[0] == 'H'), sy = (ap[0] == 'S');
   // query mode (@x@_isvd.c:199-218)
   if (argc_s < 0 || argc_o < 0 || argc_i < 0 || argc_p < 0) {
     if (argc_i < 0) {
       if (kn) kn_impl<P>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
-      else wy_impl<P>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
+      else if (!hr) wy_impl<P>(param, argv_i, argc_i, retv_i, retc_i, nullptr, 1, nullptr, 1);
     }
     return;
   }
@@ -449,9 +496,11 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   if (ao[0] == 'T') ISVD_CHECK(d.m >= d.l, "tall-skinny QR needs m >= l");
   if (ap[0] == 'T') ISVD_CHECK(d.n >= d.l, "tall-skinny QR postprocessing needs n >= l");
   if (!use_seed) ISVD_CHECK(ldo >= d.Nl, "ldomegat (%lld) must be >= Nl", ldo);
+  if (P::mixed) ISVD_CHECK(!hr && !sy, "HR integration and SY postprocessing have no mixed-precision form");
+  if (sy) ISVD_CHECK(d.m == d.n, "symmetric postprocessing needs m == n");
   KnArgs ka;
   WyArgs wa;
-  if (argc_i > 0) {
+  if (argc_i > 0 && !hr) {
     if (kn) {
       ka.maxit = (i64)argv_i[0];
       if (argc_i > 1) ka.tol = (double)argv_i[1];
@@ -513,14 +562,20 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
     else stage_integrate_wy_b(d, wa, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
   } else {
     if (kn) stage_integrate_kn<T>(d, ka, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
+    else if (hr) stage_integrate_hr<T>(d, yst.get(), d.Nl, qt.get(), d.l);
     else stage_integrate_wy<T>(d, wa, yst.get(), d.Nl, qt.get(), d.l, &it, &err);
   }
   HIP_CHECK(hipEventRecord(ev[3], ctx().stream));
   stage_msg("done.\nPostprocessing ................ ");
-  if constexpr (P::mixed)
+  if constexpr (P::mixed) {
     stage_post_b(d, ap[0] == 'T', ordera_, A.dev, lda, qt.get(), d.l, S.dev, U.dev, ldut, V.dev,
                  ldvt, want_u, want_v);
-  else if (ap[0] == 'T')
+  } else if (sy) {  // U goes to vt when ut is skipped; V = U (@x@_postprocess_symmetric.c)
+    const bool want = want_u || want_v;
+    stage_post_symmetric<T>(d, ordera_, A.dev, lda, qt.get(), d.l, S.dev,
+                            want_u ? U.dev : V.dev, want_u ? ldut : ldvt, want);
+    if (want_u && want_v) omatcopy<T>('N', d.k, d.m, T(1), U.dev, ldut, V.dev, ldvt);
+  } else if (ap[0] == 'T')
     stage_post_tsqr<T>(d, ordera_, A.dev, lda, qt.get(), d.l, S.dev, U.dev, ldut, V.dev, ldvt,
                        want_u, want_v);
   else
@@ -553,13 +608,14 @@ static void backward_impl(const isvd_Param &param, const char *alg_p, char dista
                           const typename P::T *gs, const typename P::T *gut, i64 ldgut,
                           const typename P::T *gvt, i64 ldgvt, typename P::T *da, i64 ldda) {
   using T = typename P::T;
-  const char *ap = arg2str("ALG_P", alg_p, kOptsP, 4);
+  const char *ap = arg2str("ALG_P", alg_p, kOptsP, 6);
   const char dista_ = arg2char("DISTA", dista, "CR");
   const char ordera_ = arg2char("ORDERA", ordera, "CR");
   if (!ap || !dista_ || !ordera_) return;
   check_param(param);
   const Dims d = dims_of(param);
   ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  ISVD_CHECK(ap[0] != 'S', "the adjoint covers the TS and GR postprocessing");
   ISVD_CHECK(d.n >= d.l || ap[0] != 'T', "tall-skinny QR postprocessing needs n >= l");
   const i64 rows = ordera_ == 'C' ? d.m : d.n, cols = ordera_ == 'C' ? d.n : d.m;
   ISVD_CHECK(lda >= rows, "lda (%lld) too small", lda);
@@ -744,7 +800,7 @@ void isvd_Free(void *ptr) { std::free(ptr); }
     (void)argv; (void)retv; (void)retc;                                                       \
     guarded([&] {                                                                             \
       post_impl<P>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
-                   ut_root, vt_root, true);                                                   \
+                   ut_root, vt_root, 'T');                                                    \
     });                                                                                       \
   }                                                                                           \
   void isvd_##x##PostprocessGramian(                                                           \
@@ -756,7 +812,7 @@ void isvd_Free(void *ptr) { std::free(ptr); }
     (void)argv; (void)retv; (void)retc;                                                       \
     guarded([&] {                                                                             \
       post_impl<P>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
-                   ut_root, vt_root, false);                                                  \
+                   ut_root, vt_root, 'G');                                                    \
     });                                                                                       \
   }
 
@@ -770,6 +826,37 @@ void isvd_Free(void *ptr) { std::free(ptr); }
       backward_impl<P>(param, alg_p, dista, ordera, a, lda, qt, ldqt, gs, gut, ldgut, gvt,    \
                        ldgvt, da, ldda);                                                      \
     });                                                                                       \
+  }
+
+#define ISVD_DEFINE_HR_SY(x, P, T)                                                             \
+  void isvd_##x##IntegrateHierarchicalReduction(const isvd_Param param, const T *argv,         \
+                                                const isvd_int_t argc, T *retv,                \
+                                                const isvd_int_t retc, T *yst,                 \
+                                                const isvd_int_t ldyst, T *qt,                 \
+                                                const isvd_int_t ldqt) {                       \
+    (void)argv; (void)retv; (void)retc;                                                       \
+    guarded([&] { hr_impl<P>(param, argc, yst, ldyst, qt, ldqt); });                          \
+  }                                                                                           \
+  void isvd_##x##PostprocessSymmetric(                                                         \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root) {                                                              \
+    (void)argv; (void)retv; (void)retc;                                                       \
+    guarded([&] {                                                                             \
+      post_impl<P>(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt,       \
+                   ut_root, vt_root, 'S');                                                    \
+    });                                                                                       \
+  }                                                                                           \
+  void isvd_##x##PostprocessSymmetric_gpu(                                                     \
+      const isvd_Param param, const T *argv, const isvd_int_t argc, T *retv,                  \
+      const isvd_int_t retc, const char dista, const char ordera, const T *a,                 \
+      const isvd_int_t lda, const T *qt, const isvd_int_t ldqt, T *s, T *ut,                  \
+      const isvd_int_t ldut, T *vt, const isvd_int_t ldvt, const mpi_int_t ut_root,          \
+      const mpi_int_t vt_root) {                                                              \
+    isvd_##x##PostprocessSymmetric(param, argv, argc, retv, retc, dista, ordera, a, lda, qt,  \
+                                   ldqt, s, ut, ldut, vt, ldvt, ut_root, vt_root);            \
   }
 
 #define ISVD_DEFINE_GPU_TWINS(x, T)                                                            \
@@ -806,6 +893,8 @@ ISVD_DEFINE(d, PrecD, isvd_d_val_t, isvd_d_val_t)
 ISVD_DEFINE(b, PrecB, float, isvd_b_val_t)
 ISVD_DEFINE_GPU_TWINS(s, isvd_s_val_t)
 ISVD_DEFINE_GPU_TWINS(d, isvd_d_val_t)
+ISVD_DEFINE_HR_SY(s, PrecS, isvd_s_val_t)
+ISVD_DEFINE_HR_SY(d, PrecD, isvd_d_val_t)
 ISVD_DEFINE_BACKWARD(s, PrecS, isvd_s_val_t)
 ISVD_DEFINE_BACKWARD(d, PrecD, isvd_d_val_t)
 

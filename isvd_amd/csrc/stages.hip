@@ -517,6 +517,44 @@ void stage_integrate_wy(const Dims &d, const WyArgs &args, const T *yst, i64 ldy
   gemm<T>('T', 'N', l, m, Nl, T(1), x.get(), Nl, yst, ldyst, T(0), qt, ldqt);
 }
 
+// Q_i^T rows r := tmp rows r / sqrt(2 (1 + s_i[r])) for the h merged blocks
+template <typename T>
+__global__ void hr_scale_kernel(i64 hl, i64 m, const T *tmp, const T *s, T *yst, i64 ldyst) {
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < hl * m;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 r = e % hl, c = e / hl;
+    yst[r + c * ldyst] = tmp[r + c * hl] / sqrt(T(2) * (T(1) + s[r]));
+  }
+}
+
+// Hierarchical-reduction integration (@x@_integrate_hierarchical_reduction.c:87-135): pairwise
+// merges of the sketch bases, h = floor(h2 / 2) pairs per level, all pairs of a level in one
+// batched product, one batched l x l SVD and one batched update.  yst is overwritten.
+template <typename T>
+void stage_integrate_hr(const Dims &d, T *yst, i64 ldyst, T *qt, i64 ldqt) {
+  const i64 l = d.l, N = d.N, m = d.m;
+  const i64 hmax = N / 2 > 0 ? N / 2 : 1;
+  DBuf<T> bs(l * l * hmax), wb(l * l * hmax), tb(l * l * hmax), s(l * hmax), tmp(hmax * l * m);
+  for (i64 h2 = N; h2 > 1; h2 = (h2 + 1) / 2) {
+    const i64 h = h2 / 2;
+    // B_i := Q_i^T Q_{i+h}  (:93-95)
+    gemm<T>('N', 'T', l, l, m, T(1), yst, ldyst, yst + h * l, ldyst, T(0), bs.get(), l, h, l, l,
+            l * l);
+    // svd(B_i) = W_i S_i T_i^T  (:112)
+    svd_small<T>(l, bs.get(), l, l * l, s.get(), l, wb.get(), l, l * l, tb.get(), l, l * l, h);
+    // Q_i^T := W_i^T Q_i^T + T_i^T Q_{i+h}^T  (:115-117), then / sqrt(2 (I + S_i))  (:120-122)
+    gemm<T>('T', 'N', l, m, l, T(1), wb.get(), l, yst, ldyst, T(0), tmp.get(), h * l, h, l * l,
+            l, l);
+    gemm<T>('T', 'N', l, m, l, T(1), tb.get(), l, yst + h * l, ldyst, T(1), tmp.get(), h * l, h,
+            l * l, l, l);
+    hipLaunchKernelGGL(hr_scale_kernel<T>, dim3(grid_for(h * l * m)), dim3(256), 0, ctx().stream,
+                       h * l, m, (const T *)tmp.get(), (const T *)s.get(), yst, ldyst);
+    LAUNCH_CHECK();
+  }
+  // Qbar := Q_0  (:127)
+  omatcopy<T>('N', l, m, T(1), yst, ldyst, qt, ldqt);
+}
+
 // Z^T := Qbar^T op(A)  (projectBlockRow, src/lib/libisvd/core/stage/@x@_postprocess.h:154-157)
 template <typename T>
 static void project(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
@@ -588,7 +626,27 @@ void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *q
   post_tsqr_z<T>(d, qt, ldqt, zt.get(), s, ut, ldut, vt, ldvt, want_u, want_v);
 }
 
+// Symmetric postprocessing (src/lib/libisvd/core/stage/@x@_postprocess_symmetric.c:150-184):
+// W = Z^T Qbar, syev(W) (eigenvalues ascending), U = Qbar W[:, :k].
+template <typename T>
+void stage_post_symmetric(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                          T *s, T *ut, i64 ldut, bool want_u) {
+  const i64 l = d.l, k = d.k;
+  ISVD_CHECK(d.m == d.n, "symmetric postprocessing needs m == n (m = %lld, n = %lld)", d.m, d.n);
+  DBuf<T> zt(l * d.n), w(l * l), wv(l * l);
+  project<T>(d, ordera, a, lda, qt, ldqt, zt.get());
+  // W := Z^T Q  (:154)
+  gemm<T>('N', 'T', l, l, d.n, T(1), zt.get(), l, qt, ldqt, T(0), w.get(), l);
+  // eig(W) = W S W^T  (:159)
+  syev_small<T>(l, w.get(), l, s, wv.get(), l);
+  // U^T := W[:, :k]^T Q^T  (:167)
+  if (want_u) gemm<T>('T', 'N', k, d.m, l, T(1), wv.get(), l, qt, ldqt, T(0), ut, ldut);
+}
+
 #define INST(T)                                                                                \
+  template void stage_integrate_hr<T>(const Dims &, T *, i64, T *, i64);                       \
+  template void stage_post_symmetric<T>(const Dims &, char, const T *, i64, const T *, i64,    \
+                                        T *, T *, i64, bool);                                  \
   template void gaussian_omega<T>(i64, i64, T *, i64, unsigned long long);                    \
   template void stage_sketch<T>(const Dims &, char, const T *, i64, const T *, i64, T *, i64); \
   template void stage_orth_gramian<T>(const Dims &, T *, i64);                                \

@@ -55,6 +55,15 @@ template <typename T>
 void stage_post_tsqr(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
                      T *s, T *ut, i64 ldut, T *vt, i64 ldvt, bool want_u, bool want_v);
 
+// Hierarchical-reduction integration; yst is overwritten
+template <typename T>
+void stage_integrate_hr(const Dims &d, T *yst, i64 ldyst, T *qt, i64 ldqt);
+
+// Symmetric postprocessing (m == n): s ascending (l values), U^T = first k rows of ut
+template <typename T>
+void stage_post_symmetric(const Dims &d, char ordera, const T *a, i64 lda, const T *qt, i64 ldqt,
+                          T *s, T *ut, i64 ldut, bool want_u);
+
 // The integration loops in the coordinates of the sketch basis, from M = Qs^T Qs (Nl x Nl):
 // x_out (Nl x l) receives the coefficients of Qbar = Qs x_out.
 template <typename T>

@@ -93,7 +93,13 @@ def integrate(param, qs, alg="KN", argv=None, mixed=False):
     argv = list(argv or [])
     av = (ct * max(len(argv), 1))(*argv)
     rv = (ct * 2)()
-    name = {"KN": "IntegrateKolmogorovNagumo", "WY": "IntegrateWenYin"}[alg.upper()]
+    name = {"KN": "IntegrateKolmogorovNagumo", "WY": "IntegrateWenYin",
+            "HR": "IntegrateHierarchicalReduction"}[alg.upper()]
+    if alg.upper() == "HR":  # destroys its input, as the reference's does
+        getattr(lib, f"isvd_{x}{name}")(param, None, 0, None, 0, capi.addr(qs), qs.shape[1],
+                                        capi.addr(qbar), l)
+        _check(name)
+        return qbar, 0, 0.0
     getattr(lib, f"isvd_{x}{name}")(param, ctypes.addressof(av), len(argv),
                                     ctypes.addressof(rv), 2, capi.addr(qs), qs.shape[1],
                                     capi.addr(qbar), l)
@@ -109,7 +115,8 @@ def postprocess(param, a, qbar, alg="GR", ordera="R", dista="R", want_u=True, wa
     s = _empty_like_kind(a, (l,), dt)
     u = _empty_like_kind(a, (param.nrow, l), dt) if want_u else None
     v = _empty_like_kind(a, (param.ncol, l), dt) if want_v else None
-    name = {"GR": "PostprocessGramian", "TS": "PostprocessTallSkinnyQr"}[alg.upper()]
+    name = {"GR": "PostprocessGramian", "TS": "PostprocessTallSkinnyQr",
+            "SY": "PostprocessSymmetric"}[alg.upper()]
     getattr(lib, f"isvd_{x}{name}")(
         param, None, 0, None, 0, capi._chr(dista), capi._chr(ordera), capi.addr(a),
         _a_ld(a, ordera), capi.addr(qbar), qbar.shape[1], capi.addr(s), capi.addr(u), l,

@@ -47,8 +47,9 @@ from scipy.linalg import lapack as _lp
 
 __all__ = [
     "sketch_gaussian_projection", "orthogonalize_gramian", "orthogonalize_tall_skinny_qr",
-    "integrate_kolmogorov_nagumo", "integrate_wen_yin", "postprocess_gramian",
-    "postprocess_tall_skinny_qr", "isvd", "KN_DEFAULTS", "WY_DEFAULTS",
+    "integrate_kolmogorov_nagumo", "integrate_wen_yin", "integrate_hierarchical_reduction",
+    "postprocess_gramian", "postprocess_tall_skinny_qr", "postprocess_symmetric", "isvd",
+    "KN_DEFAULTS", "WY_DEFAULTS",
 ]
 
 # Default stage arguments (src/lib/libisvd/core/stage/@x@_integrate_kolmogorov_nagumo.c:16-17,
@@ -293,6 +294,29 @@ def integrate_wen_yin(qs, num_sketch, maxit=256, tol=1e-3, tau0=1.0, taumax=1e14
     return qp, it, mu
 
 
+def integrate_hierarchical_reduction(qs, num_sketch):
+    """Hierarchical-reduction integration; returns Qbar (m x l).
+
+    src/lib/libisvd/core/stage/@x@_integrate_hierarchical_reduction.c:87-129: for
+    h2 = N, ceil(h2/2), ... while h2 > 1, with h = floor(h2/2) and every i < h:
+    B_i = Q_i^T Q_{i+h}; gesvd('O','S'): B_i = W S T^T; Q_i := (Q_i W + Q_{i+h} T) / sqrt(2(I+S))
+    (column scaling); Qbar = Q_0.  For odd h2 the block h2 - 1 takes no part in that level and
+    the next level reads blocks 0 .. ceil(h2/2) - 1, as the reference's loop does.
+    """
+    m, nl = qs.shape
+    l = nl // num_sketch
+    q = [qs[:, i * l:(i + 1) * l].copy() for i in range(num_sketch)]
+    h2 = num_sketch
+    while h2 > 1:
+        h = h2 // 2
+        for i in range(h):
+            b = q[i].T @ q[i + h]
+            w, s, tt = _gesvd_full(b)
+            q[i] = (q[i] @ w + q[i + h] @ tt.T) / np.sqrt(2.0 * (1.0 + s))[None, :]
+        h2 = (h2 + 1) // 2
+    return q[0]
+
+
 # --------------------------------------------------------------------------------------------
 # Postprocessing
 # --------------------------------------------------------------------------------------------
@@ -338,6 +362,20 @@ def postprocess_tall_skinny_qr(a, qbar, rank, want_u=True, want_v=True, referenc
         u = qbar @ u_l[:, :rank] if want_u else None
         v = q_qr @ vt_l[:rank, :].T if want_v else None
     return s, u, v
+
+
+def postprocess_symmetric(a, qbar, rank, want_u=True):
+    """Symmetric postprocessing (A = A^T); returns (s[l] ascending, U[m,k] or None).
+
+    src/lib/libisvd/core/stage/@x@_postprocess_symmetric.c:150-184: Z^T = Qbar^T A; W = Z^T Qbar
+    (upper triangle, gemmt); syev(W) = W diag(s) W^T, eigenvalues ascending; U = Qbar W[:, :k].
+    V is U.
+    """
+    zt = qbar.T @ a
+    w = zt @ qbar
+    s, wv = _syev_upper(np.ascontiguousarray(w))
+    u = qbar @ wv[:, :rank] if want_u else None
+    return s, u
 
 
 # --------------------------------------------------------------------------------------------

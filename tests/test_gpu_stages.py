@@ -166,3 +166,88 @@ def test_postprocess_k_less_than_l_matches_textbook_svd(torch_cuda, alg):
     assert np.abs(host(s) - s_ref).max() < 1e-10 * s_ref[0]
     assert principal_angles(host(u)[:, :6], u_ref).max() < 1e-8
     assert principal_angles(host(v)[:, :6], v_ref).max() < 1e-8
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_hr_golden(golden, torch_cuda, dtype):
+    # check/.../@x@_integrate_hierarchical_reduction.cxx: projector of Qbar vs q_hr.mtx
+    p = capi.create_param(101, 101, 11, 0, 12)
+    q, _, _ = pipeline.integrate(p, dev(torch_cuda, golden["qs"], dtype), "HR")
+    assert np.abs(proj(host(q)) - proj(golden["q_hr"])).max() < TOL[dtype]
+
+
+def test_hr_matches_oracle_l128(torch_cuda):
+    # odd level sizes (N = 7: 7 -> 4 -> 2 -> 1) follow the reference's loop
+    from conftest import lowrank_matrix
+    m, l, N = 3000, 128, 7
+    a = lowrank_matrix(m, 900, 0.97 ** np.arange(900), seed=12)
+    y = a @ np.random.default_rng(13).standard_normal((900, N * l))
+    qs = O.orthogonalize_tall_skinny_qr(y, N)
+    p = capi.create_param(m, m, 100, 28, N)
+    q, _, _ = pipeline.integrate(p, dev(torch_cuda, qs, np.float64), "HR")
+    q_ref = O.integrate_hierarchical_reduction(qs, N)
+    assert np.abs(proj(host(q)) - proj(q_ref)).max() < 1e-10
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("ordera", ["C", "R"])
+@pytest.mark.parametrize("job", ["gather", "u_only", "v_only"])
+def test_postprocess_symmetric_golden(golden, torch_cuda, dtype, ordera, job):
+    # check/.../@x@_postprocess_symmetric.hxx: s vs s_sym.mtx (ascending), U projector
+    a, qbar = golden["a_sym"], golden["q"]
+    m = a.shape[0]
+    p = capi.create_param(m, m, 11, 0, 1)
+    lib = capi.load()
+    torch = torch_cuda
+    tdt = torch.float64 if dtype == np.float64 else torch.float32
+    a_arg = dev(torch, a if ordera == "R" else a.T, dtype)
+    q_arg = dev(torch, qbar, dtype)
+    s = torch.zeros(11, dtype=tdt, device="cuda")
+    u = torch.zeros((m, 11), dtype=tdt, device="cuda")
+    v = torch.zeros((m, 11), dtype=tdt, device="cuda")
+    ur, vr = {"gather": (0, 0), "u_only": (0, -2), "v_only": (-2, 0)}[job]
+    x = "d" if dtype == np.float64 else "s"
+    getattr(lib, f"isvd_{x}PostprocessSymmetric")(
+        p, None, 0, None, 0, b"R", ordera.encode(), a_arg.data_ptr(), m, q_arg.data_ptr(), 11,
+        s.data_ptr(), u.data_ptr(), 11, v.data_ptr(), 11, ur, vr)
+    assert capi.last_error() == ""
+    tol = TOL[dtype] * (1 if dtype == np.float64 else golden["s_sym"].max())
+    assert np.abs(host(s) - golden["s_sym"][:, 0]).max() < tol
+    ptol = TOL[dtype]
+    if job in ("gather", "u_only"):
+        assert np.abs(proj(host(u)) - proj(golden["u_sym"])).max() < ptol
+    if job in ("gather", "v_only"):
+        assert np.abs(proj(host(v)) - proj(golden["u_sym"])).max() < ptol
+    if job == "u_only":
+        assert float(v.abs().max()) == 0.0
+
+
+@pytest.mark.parametrize("algs", [("GP", "TS", "HR", "TS"), ("GP", "GR", "KN", "SY")])
+def test_driver_hr_and_sy_match_oracle(torch_cuda, algs):
+    from conftest import lowrank_matrix
+    m, k, p, N = 600, 10, 6, 4
+    if algs[3] == "SY":
+        rng = np.random.default_rng(17)
+        vq, _ = np.linalg.qr(rng.standard_normal((m, m)))
+        a = (vq * (0.9 ** np.arange(m))[None, :]) @ vq.T
+        n = m
+    else:
+        n = 400
+        a = lowrank_matrix(m, n, 0.9 ** np.arange(n), seed=18)
+    om = np.random.default_rng(19).standard_normal((n, (k + p) * N))
+    prm = capi.create_param(m, n, k, p, N)
+    r = pipeline.isvd(prm, dev(torch_cuda, a, np.float64), omega=dev(torch_cuda, om, np.float64),
+                      algs=algs)
+    y = O.sketch_gaussian_projection(a, om)
+    qs = (O.orthogonalize_tall_skinny_qr if algs[1] == "TS" else O.orthogonalize_gramian)(y, N)
+    if algs[2] == "HR":
+        qbar = O.integrate_hierarchical_reduction(qs, N)
+    else:
+        qbar, _, _ = O.integrate_kolmogorov_nagumo(qs, N)
+    if algs[3] == "SY":
+        s_ref, u_ref = O.postprocess_symmetric(a, qbar, k)
+    else:
+        s_ref, u_ref, _ = O.postprocess_tall_skinny_qr(a, qbar, k)
+    assert np.abs(host(r["s"]) - s_ref).max() < 1e-10 * np.abs(s_ref).max()
+    from conftest import principal_angles
+    assert principal_angles(host(r["u"])[:, :k], u_ref).max() < 1e-8

@@ -109,6 +109,60 @@ def test_kn_maxit_exhaustion_reports_maxit_plus_one(golden):
 
 
 # ---- postprocessing ------------------------------------------------------------------------
+def test_hr_golden(golden):
+    # check/.../@x@_integrate_hierarchical_reduction.cxx: projector of Qbar vs q_hr.mtx
+    q = O.integrate_hierarchical_reduction(golden["qs"], 12)
+    assert np.abs(proj(q) - proj(golden["q_hr"])).max() < DERR
+    assert np.abs(q.T @ q - np.eye(11)).max() < 1e-13
+
+
+def test_hr_single_sketch_and_common_subspace():
+    rng = np.random.default_rng(4)
+    q0, _ = np.linalg.qr(rng.standard_normal((300, 9)))
+    assert np.array_equal(O.integrate_hierarchical_reduction(q0, 1), q0)
+    blocks = []
+    for _ in range(4):  # the same subspace in 4 different orthonormal bases
+        r, _ = np.linalg.qr(rng.standard_normal((9, 9)))
+        blocks.append(q0 @ r)
+    q = O.integrate_hierarchical_reduction(np.concatenate(blocks, axis=1), 4)
+    assert np.abs(proj(q) - proj(q0)).max() < 1e-13
+
+
+def test_hr_pair_merge_closed_form():
+    """Two subspaces at principal angles theta_j: the merged basis spans the bisectors
+    (w_j + t_j) / |w_j + t_j| of the principal vector pairs."""
+    rng = np.random.default_rng(5)
+    m, l = 200, 6
+    q1, _ = np.linalg.qr(rng.standard_normal((m, l)))
+    q2, _ = np.linalg.qr(q1 + 0.3 * rng.standard_normal((m, l)))
+    u, s, vt = np.linalg.svd(q1.T @ q2)
+    p1, p2 = q1 @ u, q2 @ vt.T
+    bis = (p1 + p2) / np.linalg.norm(p1 + p2, axis=0)[None, :]
+    q = O.integrate_hierarchical_reduction(np.concatenate([q1, q2], axis=1), 2)
+    assert np.abs(proj(q) - proj(bis)).max() < 1e-13
+
+
+def test_sy_golden(golden):
+    # check/.../@x@_postprocess_symmetric.hxx: s vs s_sym.mtx entrywise, U projector (k = l)
+    s, u = O.postprocess_symmetric(golden["a_sym"], golden["q"], 11)
+    assert np.abs(s - golden["s_sym"].ravel()).max() < DERR
+    assert np.abs(proj(u) - proj(golden["u_sym"])).max() < DERR
+
+
+def test_sy_exact_invariant_subspace():
+    rng = np.random.default_rng(6)
+    n, l = 120, 7
+    v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    lam = np.linspace(-3.0, 5.0, n)
+    a = (v * lam[None, :]) @ v.T
+    r, _ = np.linalg.qr(rng.standard_normal((l, l)))
+    qbar = v[:, 10:10 + l] @ r
+    s, u = O.postprocess_symmetric(a, qbar, l)
+    assert np.abs(s - np.sort(lam[10:10 + l])).max() < 1e-12
+    # eigenvalues ascending, eigenvectors the matching columns of V (up to sign)
+    assert np.abs(np.abs(u.T @ v[:, 10:10 + l]) - np.eye(l)).max() < 1e-10
+
+
 @pytest.mark.parametrize("fn", [O.postprocess_gramian, O.postprocess_tall_skinny_qr])
 def test_post_golden(golden, fn):
     # check/.../@x@_postprocess_gramian.hxx:270-300 (k = l, projector comparison)
