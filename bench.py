@@ -100,8 +100,9 @@ def principal_angle_max(torch, a, b):
 
 
 def mixed_case(torch, capi, pipeline, m, n, k, p, N, r, decay, steps, warmup):
-    """BF16 A (spectrum 0.95^i or i^-0.5, + 1e-6 noise) through the mixed-precision driver,
-    timed, and compared with the FP64 driver on the identical BF16 values of A and Omega."""
+    """BF16 A (spectrum 0.95^i or i^-0.5, + 1e-6 noise) through the mixed-precision driver
+    (GP / GR twice in FP32 / KN / TS), timed, and compared with the FP64 driver (same selectors)
+    on the identical BF16 values of A and Omega."""
     import numpy as np
     i = np.arange(r, dtype=np.float64)
     spectrum = 0.95 ** i if decay == "fast" else (i + 1.0) ** -0.5
@@ -111,8 +112,9 @@ def mixed_case(torch, capi, pipeline, m, n, k, p, N, r, decay, steps, warmup):
     gen = torch.Generator(device="cuda").manual_seed(77)
     om_b = torch.randn(n, (k + p) * N, generator=gen, device="cuda",
                        dtype=torch.float32).to(torch.bfloat16)
+    torch.cuda.synchronize()
     prm = capi.create_param(m, n, k, p, N)
-    algs = ("GP", "TS", "KN", "TS")
+    algs = ("GP", "GR", "KN", "TS")  # the Gramian (applied twice in FP32) orthogonalizes
     for _ in range(warmup):
         rb = pipeline.isvd(prm, a_b, omega=om_b, algs=algs)
     torch.cuda.synchronize()
@@ -262,8 +264,8 @@ def main():
         del a
         torch.cuda.empty_cache()
         mixed = {"workload": f"A {m}x{n} stored in BF16 (U diag(s) V^T + 1e-6 noise, rank {r}), "
-                             f"k={k} p={p} N={N}, GP/TS/KN/TS; BF16 MFMA sketch and projection, "
-                             f"split-BF16 x3 Gram, FP64 l x l solves; errors vs the FP64 driver "
+                             f"k={k} p={p} N={N}, GP/GR/KN/TS; BF16 MFMA sketch and projection, "
+                             f"Gramian twice in FP32, split-BF16 x3 Gram, FP64 l x l solves; errors vs the FP64 driver "
                              f"on the same BF16 values",
                  "cases": [mixed_case(torch, capi, pipeline, m, n, k, p, N, r, decay,
                                       args.steps, args.warmup) for decay in ("fast", "slow")]}

@@ -55,7 +55,9 @@ void ensure_init() {
   }
   HIP_CHECK(hipSetDevice(0));
   g_ctx.device = 0;
-  HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+  // A blocking stream: work the caller queued on the legacy default stream (e.g. the kernels
+  // that produced device-resident inputs) completes before the library's kernels start.
+  HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamDefault));
   g_init = true;
 }
 

@@ -203,6 +203,7 @@ struct Cfg {
   static constexpr int FM = BM / WGM / 16, FN = BN / WGN / 16;
 };
 using Cfg256 = Cfg<256, 256, 2, 4, 64>;  // 8 waves of 128 x 64
+using CfgBig = Cfg<256, 256, 2, 2, 64>;  // 4 waves of 128 x 128 (accumulators in AGPRs)
 using CfgTall = Cfg<256, 128, 4, 2, 64>;  // 8 waves of 64 x 64
 using Cfg128 = Cfg<128, 128, 2, 2, 64>;  // 4 waves of 64 x 64
 
@@ -485,7 +486,7 @@ void gemm_mx(const MxOperand &a, const MxOperand &b, i64 m, i64 n, i64 k, float 
   const int tile = env_int("ISVD_MX_TILE", 256);
   const bool wide = (m >= 256 && n >= 256);
   if (!a.f32 && !b.f32) {  // sketch: BF16 x BF16
-    const int cfg = !wide ? 0 : (tile == 256 ? 2 : tile == 128 ? 0 : 1);
+    const int cfg = !wide ? 0 : (tile == 256 ? 2 : tile == 128 ? 0 : tile == 2562 ? 3 : 1);
     const int code = (a.kc ? 2 : 0) + (b.kc ? 1 : 0) + 4 * cfg;
     switch (code) {
       case 0: launch_mx<h16, false, 1, h16, false, 1, Cfg128>(g, batch); break;
@@ -499,7 +500,11 @@ void gemm_mx(const MxOperand &a, const MxOperand &b, i64 m, i64 n, i64 k, float 
       case 8: launch_mx<h16, false, 1, h16, false, 1, Cfg256>(g, batch); break;
       case 9: launch_mx<h16, false, 1, h16, true, 1, Cfg256>(g, batch); break;
       case 10: launch_mx<h16, true, 1, h16, false, 1, Cfg256>(g, batch); break;
-      default: launch_mx<h16, true, 1, h16, true, 1, Cfg256>(g, batch); break;
+      case 11: launch_mx<h16, true, 1, h16, true, 1, Cfg256>(g, batch); break;
+      case 12: launch_mx<h16, false, 1, h16, false, 1, CfgBig>(g, batch); break;
+      case 13: launch_mx<h16, false, 1, h16, true, 1, CfgBig>(g, batch); break;
+      case 14: launch_mx<h16, true, 1, h16, false, 1, CfgBig>(g, batch); break;
+      default: launch_mx<h16, true, 1, h16, true, 1, CfgBig>(g, batch); break;
     }
   } else if (a.f32 && !a.kc && a.parts == 2 && b.f32 && !b.kc && b.parts == 2) {
     // Gram of the stacked bases

@@ -41,6 +41,15 @@ def _empty_like_kind(ref, shape, dtype):
     return np.empty(shape, dtype=dtype)
 
 
+def _ready(*arrays):
+    """Wait for the caller's stream to finish producing device inputs: the library runs on its
+    own stream, which is ordered after the legacy default stream but not after other streams."""
+    for x in arrays:
+        if _is_torch(x) and x.is_cuda:
+            _torch().cuda.current_stream(x.device).synchronize()
+            return
+
+
 def _check(fn_name):
     err = capi.last_error()
     if err:
@@ -55,6 +64,7 @@ def _a_ld(a, ordera):
 def sketch(param, a, omega=None, seed=None, ordera="R", dista="R"):
     """Y = A Omega.  With ``omega`` (n x Nl row-major) uses the caller's draw, else ``seed``."""
     lib = capi.load()
+    _ready(a, omega)
     x, dt = _family(a)
     m, Nl = param.nrow, param.dim_sketch_total
     y = _empty_like_kind(a, (m, Nl), dt)
@@ -75,6 +85,7 @@ def orthogonalize(param, y, alg="GR", mixed=False):
     """In-place orthogonalization of every sketch block of Y (m x Nl); ``mixed`` selects the
     b family (FP32 Y, Gramian twice)."""
     lib = capi.load()
+    _ready(y)
     x, _ = _family(y, mixed)
     name = {"GR": "OrthogonalizeGramian", "TS": "OrthogonalizeTallSkinnyQr"}[alg.upper()]
     getattr(lib, f"isvd_{x}{name}")(param, None, 0, None, 0, capi.addr(y), y.shape[1])
@@ -86,6 +97,7 @@ def integrate(param, qs, alg="KN", argv=None, mixed=False):
     """Returns (Qbar m x l, iter, err_or_mu); ``mixed`` selects the b family (FP32 bases,
     split-BF16 Gram)."""
     lib = capi.load()
+    _ready(qs)
     x, dt = _family(qs, mixed)
     ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
     l = param.dim_sketch
@@ -110,6 +122,7 @@ def integrate(param, qs, alg="KN", argv=None, mixed=False):
 def postprocess(param, a, qbar, alg="GR", ordera="R", dista="R", want_u=True, want_v=True):
     """Returns (s[l], U m x l or None, V n x l or None); the first k columns of U/V are valid."""
     lib = capi.load()
+    _ready(a, qbar)
     x, dt = _family(a)
     l = param.dim_sketch
     s = _empty_like_kind(a, (l,), dt)
@@ -130,6 +143,7 @@ def isvd(param, a, omega=None, seed=0, algs=("GP", "GR", "KN", "GR"), ordera="R"
     """Full driver (isvd_?Isvd / isvd_?IsvdOmega).  Returns dict(s, u, v, iter, err, time).
     A BF16 ``a`` (torch.bfloat16 or uint16 bits) selects the mixed-precision b family."""
     lib = capi.load()
+    _ready(a, omega)
     x, dt = _family(a)
     ct = ctypes.c_double if dt == np.float64 else ctypes.c_float
     l = param.dim_sketch
@@ -158,6 +172,7 @@ def isvd_backward(param, a, qbar, gs=None, gu=None, gv=None, alg_p="TS", ordera=
     (m x l), ``gs`` (k), ``gu`` (m x k) and ``gv`` (n x k) the gradients of the outputs (None for
     zero).  Returns dL/dA with the shape and storage of ``a``."""
     lib = capi.load()
+    _ready(a, qbar, gs, gu, gv)
     x, dt = _family(a)
     if x == "b":
         raise ValueError("isvd_backward: the mixed-precision family has no adjoint")
