@@ -63,6 +63,41 @@ void ensure_init() {
 
 void sync() { HIP_CHECK(hipStreamSynchronize(g_ctx.stream)); }
 
+static void *g_ws = nullptr;
+static size_t g_ws_bytes = 0;
+
+void *splitk_workspace(size_t bytes) {
+  if (bytes > g_ws_bytes) {
+    if (g_ws) HIP_CHECK(hipFreeAsync(g_ws, g_ctx.stream));
+    g_ws = nullptr;
+    g_ws_bytes = 0;
+    const size_t grown = bytes + bytes / 4;
+    HIP_CHECK(hipMallocAsync(&g_ws, grown, g_ctx.stream));
+    g_ws_bytes = grown;
+  }
+  return g_ws;
+}
+
+static void *g_pinned = nullptr;
+static hipEvent_t g_readback_ev = nullptr;
+constexpr size_t kPinnedBytes = 4096;
+
+void read_back(void *host, const void *dev, size_t bytes) {
+  ISVD_CHECK(bytes <= kPinnedBytes, "read_back: %zu bytes is too many", bytes);
+  if (!g_pinned) {
+    HIP_CHECK(hipHostMalloc(&g_pinned, kPinnedBytes, hipHostMallocDefault));
+    HIP_CHECK(hipEventCreateWithFlags(&g_readback_ev, hipEventDisableTiming));
+  }
+  HIP_CHECK(hipMemcpyAsync(g_pinned, dev, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+  HIP_CHECK(hipEventRecord(g_readback_ev, g_ctx.stream));
+  for (;;) {
+    const hipError_t e = hipEventQuery(g_readback_ev);
+    if (e == hipSuccess) break;
+    if (e != hipErrorNotReady) HIP_CHECK(e);
+  }
+  std::memcpy(host, g_pinned, bytes);
+}
+
 bool is_device_pointer(const void *ptr) {
   if (!ptr) return false;
   hipPointerAttribute_t attr;
@@ -690,6 +725,10 @@ void isvd_Finalize_gpu(void) {
   guarded([] {
     if (g_init) {
       HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+      if (g_ws) HIP_CHECK(hipFreeAsync(g_ws, g_ctx.stream));
+      HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
+      g_ws = nullptr;
+      g_ws_bytes = 0;
       HIP_CHECK(hipStreamDestroy(g_ctx.stream));
       g_ctx.stream = nullptr;
       g_init = false;

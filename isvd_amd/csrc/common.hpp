@@ -93,6 +93,14 @@ struct DBuf {
   T *get() const { return p; }
 };
 
+// Split-K partial sums: one device buffer per library context, grown on demand and reused by
+// every product (all work is ordered on ctx().stream, so consecutive products never overlap).
+void *splitk_workspace(size_t bytes);
+
+// Host <- device copy of a few values that returns as soon as they land: the copy goes to
+// pinned host memory and the host spins on an event instead of a blocking stream wait.
+void read_back(void *host, const void *dev, size_t bytes);
+
 // True when ptr is device (HBM) memory usable in place by kernels on the current device.
 bool is_device_pointer(const void *ptr);
 

@@ -293,6 +293,9 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
   if (busy * batch < kTargetBlocks && g.k >= 8 * BK) {
     i64 want = (kTargetBlocks + busy * batch - 1) / (busy * batch);
     i64 maxs = g.k / (4 * BK);
+    // partial sums cost ksplit x the output in HBM traffic: keep them within ~64 MB
+    const i64 by_traffic = (i64)(64ll << 20) / ((i64)sizeof(T) * g.m * g.n * batch);
+    if (maxs > (by_traffic > 4 ? by_traffic : 4)) maxs = by_traffic > 4 ? by_traffic : 4;
     if (want > maxs) want = maxs;
     if (want > 64) want = 64;
     if (want < 1) want = 1;
@@ -305,11 +308,8 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
   g.ksplit = ksplit;
   g.kchunk = (ksplit == 1) ? (g.k > 0 ? g.k : 1) : kchunk;
   g.group_m = 8;
-  DBuf<T> work;
-  if (ksplit > 1) {
-    work.alloc(batch * ksplit * g.m * g.n);
-    g.W = work.get();
-  }
+  if (ksplit > 1)
+    g.W = static_cast<T *>(splitk_workspace(sizeof(T) * (size_t)(batch * ksplit * g.m * g.n)));
   ISVD_CHECK(batch * ksplit <= 65535, "grid.y too large (batch %lld x ksplit %d)", batch, ksplit);
   dim3 grid((unsigned)tiles, (unsigned)(batch * ksplit));
   hipLaunchKernelGGL((gemm_kernel<T, TA, TB, BM, BN, BK, WGM, WGN>), grid, dim3(64 * WGM * WGN),

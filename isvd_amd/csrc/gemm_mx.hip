@@ -1,13 +1,13 @@
 // BF16 MFMA GEMM family for gfx950 with FP32 accumulation (v_mfma_f32_16x16x32_bf16).
 //
-// Workgroup tiles 256 x 256 (8 waves of 128 x 64) or, for the narrow l x n projection,
-// 128 x 128 (4 waves of 64 x 64); K step 64.  A 256 x 256 BF16 tile does 128 FLOP per byte it
-// reads through L2, twice the 128 x 128 tile's, which at the BF16 matrix peak would need all of
-// the L2 bandwidth.  Both operands are staged in LDS as [mn][k] rows of 64 BF16 values (+8
-// padding), so every MFMA fragment (lane -> mn = lane%16, k = 8*(lane/16) .. +7) is one 16-byte
-// LDS read, conflict-free across each 16-lane group (row stride 36 dwords).
+// Used for the two fused split products of the mixed pipeline (the plain BF16 sketch goes
+// through hipBLASLt): the Gram of the stacked FP32 bases (256 x 128 tiles, 8 waves of 64 x 64)
+// and the projection Qbar^T A (128 x 128 tiles, 4 waves of 64 x 64); K step 64.  Both operands
+// are staged in LDS as [mn][k] rows of 64 BF16 values (+8 padding), so every MFMA fragment
+// (lane -> mn = lane%16, k = 8*(lane/16) .. +7) is one 16-byte LDS read.  Two LDS stages when
+// they fit (global loads run two K steps ahead, one barrier per step), one otherwise.
 //  - K-contiguous sources are copied 16 bytes at a time.
-//  - MN-contiguous sources (the sketch's Omega^T and A^T, the Gram of the stacked bases) are
+//  - MN-contiguous sources (the Gram of the stacked bases, Qbar^T of the projection) are
 //    loaded as 8 consecutive mn at 4 consecutive k per thread and transposed in registers into
 //    8-byte [mn][k..k+3] LDS writes; lanes 0..15 of a wave write one 128-byte LDS row span, so
 //    the transpose writes are conflict-free too.
@@ -201,9 +201,16 @@ struct Cfg {
   static constexpr int BM = BM_, BN = BN_, WGM = WGM_, WGN = WGN_, KT = KT_;
   static constexpr int NT = 64 * WGM * WGN;
   static constexpr int FM = BM / WGM / 16, FN = BN / WGN / 16;
+  // LDS bytes of one stage for PA / PB BF16 parts per operand
+  template <int QA, int QB>
+  static constexpr int stage_bytes() {
+    return (QA * BM + QB * BN) * (KT + 8) * 2;
+  }
+  template <int QA, int QB>
+  static constexpr bool kTwoStages() {
+    return 2 * stage_bytes<QA, QB>() <= 160 * 1024;
+  }
 };
-using Cfg256 = Cfg<256, 256, 2, 4, 64>;  // 8 waves of 128 x 64
-using CfgBig = Cfg<256, 256, 2, 2, 64>;  // 4 waves of 128 x 128 (accumulators in AGPRs)
 using CfgTall = Cfg<256, 128, 4, 2, 64>;  // 8 waves of 64 x 64
 using Cfg128 = Cfg<128, 128, 2, 2, 64>;  // 4 waves of 64 x 64
 
@@ -213,8 +220,10 @@ __global__ __launch_bounds__(C::NT) void gemm_mx_kernel(MxArgs g) {
   constexpr int kBM = C::BM, kBN = C::BN, kBK = C::KT, kLK = C::KT + 8;
   constexpr int FM = C::FM, FN = C::FN;
   extern __shared__ __align__(16) unsigned char smem_raw[];
-  uint16_t *sA = reinterpret_cast<uint16_t *>(smem_raw);  // [PA][kBM][kLK]
-  uint16_t *sB = sA + PA * kBM * kLK;                      // [PB][kBN][kLK]
+  // one LDS stage: [PA][kBM][kLK] then [PB][kBN][kLK]; two stages when they fit
+  constexpr int kStage = PA * kBM * kLK + PB * kBN * kLK;
+  constexpr bool kTwo = C::template kTwoStages<PA, PB>();
+  uint16_t *const s0 = reinterpret_cast<uint16_t *>(smem_raw);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % C::WGM, wn = wave / C::WGM;
 
@@ -265,21 +274,9 @@ __global__ __launch_bounds__(C::NT) void gemm_mx_kernel(MxArgs g) {
 #pragma unroll
     for (int j = 0; j < FN; ++j) acc[i][j] = f4v{0.f, 0.f, 0.f, 0.f};
 
-  if (nk > 0) {
-    ta.load(A, g.lda, m0, g.m, kbeg, kend, g.avec, tid);
-    tb.load(B, g.ldb, n0, g.n, kbeg, kend, g.bvec, tid);
-  }
   const int fr = lane & 15, fk = 8 * (lane >> 4);
   const int wrow = wm * (kBM / C::WGM), wcol = wn * (kBN / C::WGN);
-  for (int kt = 0; kt < nk; ++kt) {
-    ta.store(sA, tid);
-    tb.store(sB, tid);
-    __syncthreads();
-    if (kt + 1 < nk) {
-      const i64 k0 = kbeg + (i64)(kt + 1) * kBK;
-      ta.load(A, g.lda, m0, g.m, k0, kend, g.avec, tid);
-      tb.load(B, g.ldb, n0, g.n, k0, kend, g.bvec, tid);
-    }
+  auto compute = [&](const uint16_t *sA, const uint16_t *sB) {
 #pragma unroll
     for (int s = 0; s < kBK / 32; ++s) {
       bf16x8 bf[PB][FN];
@@ -305,7 +302,41 @@ __global__ __launch_bounds__(C::NT) void gemm_mx_kernel(MxArgs g) {
                                                                        acc[ti][tj], 0, 0, 0);
       }
     }
+  };
+  auto load = [&](int kt) {
+    const i64 k0 = kbeg + (i64)kt * kBK;
+    ta.load(A, g.lda, m0, g.m, k0, kend, g.avec, tid);
+    tb.load(B, g.ldb, n0, g.n, k0, kend, g.bvec, tid);
+  };
+  auto store = [&](uint16_t *st) {
+    ta.store(st, tid);
+    tb.store(st + PA * kBM * kLK, tid);
+  };
+  if (kTwo) {
+    // two LDS stages: the global loads of step kt + 2 are issued while step kt computes, and
+    // one barrier per step separates the stage being read from the stage being written
+    if (nk > 0) {
+      load(0);
+      store(s0);
+    }
     __syncthreads();
+    if (nk > 1) load(1);
+    for (int kt = 0; kt < nk; ++kt) {
+      const uint16_t *cur = s0 + (kt & 1) * kStage;
+      compute(cur, cur + PA * kBM * kLK);
+      if (kt + 1 < nk) store(s0 + ((kt + 1) & 1) * kStage);
+      if (kt + 2 < nk) load(kt + 2);
+      __syncthreads();
+    }
+  } else {
+    if (nk > 0) load(0);
+    for (int kt = 0; kt < nk; ++kt) {
+      store(s0);
+      __syncthreads();
+      if (kt + 1 < nk) load(kt + 1);
+      compute(s0, s0 + PA * kBM * kLK);
+      __syncthreads();
+    }
   }
 
   // ---- epilogue: D register q of a lane holds row 4*(lane/16) + q, column lane%16
@@ -381,7 +412,7 @@ template <typename TA, bool AK, int PA, typename TB, bool BK, int PB, class C>
 void launch_mx(MxArgs g, i64 batch) {
   constexpr int kBM = C::BM, kBN = C::BN, kBK = C::KT;
   auto kern = gemm_mx_kernel<TA, AK, PA, TB, BK, PB, C>;
-  const int lds = (PA * kBM + PB * kBN) * (kBK + 8) * 2;
+  const int lds = C::template stage_bytes<PA, PB>() * (C::template kTwoStages<PA, PB>() ? 2 : 1);
   static std::once_flag once;
   std::call_once(once, [&] {
     HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -414,11 +445,9 @@ void launch_mx(MxArgs g, i64 batch) {
   g.ksplit = ksplit;
   g.kchunk = (ksplit == 1) ? (g.k > 0 ? g.k : 1) : kchunk;
   g.group_m = 8;
-  DBuf<float> work;
-  if (ksplit > 1) {
-    work.alloc(batch * ksplit * g.m * g.n);
-    g.W = work.get();
-  }
+  if (ksplit > 1)
+    g.W = static_cast<float *>(
+        splitk_workspace(sizeof(float) * (size_t)(batch * ksplit * g.m * g.n)));
   ISVD_CHECK(batch * ksplit <= 65535, "grid.y too large (batch %lld x ksplit %d)", batch, ksplit);
   ISVD_CHECK(tiles <= 0x7fffffff, "too many tiles (%lld)", tiles);
   hipLaunchKernelGGL(kern, dim3((unsigned)tiles, (unsigned)(batch * ksplit)), dim3(C::NT), lds,
@@ -430,32 +459,6 @@ void launch_mx(MxArgs g, i64 batch) {
                        g.ldc, g.sC);
     LAUNCH_CHECK();
   }
-}
-
-// dst (cols x rows, ldd) := src (rows x cols, lds)^T for 16-bit elements, 64 x 64 LDS tiles
-__global__ __launch_bounds__(256) void transpose16_kernel(i64 rows, i64 cols, const uint16_t *src,
-                                                          i64 lds, uint16_t *dst, i64 ldd) {
-  __shared__ uint16_t t[64][66];
-  const i64 i0 = blockIdx.x * 64ll, j0 = blockIdx.y * 64ll;
-  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int jj = ty + 4 * r;
-    const i64 i = i0 + tx, j = j0 + jj;
-    t[jj][tx] = (i < rows && j < cols) ? src[i + j * lds] : 0;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int ii = ty + 4 * r;
-    const i64 j = j0 + tx, i = i0 + ii;
-    if (i < rows && j < cols) dst[j + i * ldd] = t[tx][ii];
-  }
-}
-
-int env_int(const char *name, int dflt) {
-  const char *e = std::getenv(name);
-  return e ? std::atoi(e) : dflt;
 }
 
 bool aligned16(const void *p, i64 ld, i64 stride, i64 esize) {
@@ -481,32 +484,10 @@ void gemm_mx(const MxOperand &a, const MxOperand &b, i64 m, i64 n, i64 k, float 
   g.avec = aligned16(a.p, a.ld, a.stride, a.f32 ? 4 : 2);
   g.bvec = aligned16(b.p, b.ld, b.stride, b.f32 ? 4 : 2);
   using h16 = uint16_t;
-  // 256 x 256 tiles (8 waves) when both output dimensions can use them; the l x n projection
-  // (m = l <= 128) runs 128 x 128 tiles
-  const int tile = env_int("ISVD_MX_TILE", 256);
+  // 256 x 128 tiles (8 waves) for the Nl x Nl Gram; the l x n projection (m = l <= 128) runs
+  // 128 x 128 tiles
   const bool wide = (m >= 256 && n >= 256);
-  if (!a.f32 && !b.f32) {  // sketch: BF16 x BF16
-    const int cfg = !wide ? 0 : (tile == 256 ? 2 : tile == 128 ? 0 : tile == 2562 ? 3 : 1);
-    const int code = (a.kc ? 2 : 0) + (b.kc ? 1 : 0) + 4 * cfg;
-    switch (code) {
-      case 0: launch_mx<h16, false, 1, h16, false, 1, Cfg128>(g, batch); break;
-      case 1: launch_mx<h16, false, 1, h16, true, 1, Cfg128>(g, batch); break;
-      case 2: launch_mx<h16, true, 1, h16, false, 1, Cfg128>(g, batch); break;
-      case 3: launch_mx<h16, true, 1, h16, true, 1, Cfg128>(g, batch); break;
-      case 4: launch_mx<h16, false, 1, h16, false, 1, CfgTall>(g, batch); break;
-      case 5: launch_mx<h16, false, 1, h16, true, 1, CfgTall>(g, batch); break;
-      case 6: launch_mx<h16, true, 1, h16, false, 1, CfgTall>(g, batch); break;
-      case 7: launch_mx<h16, true, 1, h16, true, 1, CfgTall>(g, batch); break;
-      case 8: launch_mx<h16, false, 1, h16, false, 1, Cfg256>(g, batch); break;
-      case 9: launch_mx<h16, false, 1, h16, true, 1, Cfg256>(g, batch); break;
-      case 10: launch_mx<h16, true, 1, h16, false, 1, Cfg256>(g, batch); break;
-      case 11: launch_mx<h16, true, 1, h16, true, 1, Cfg256>(g, batch); break;
-      case 12: launch_mx<h16, false, 1, h16, false, 1, CfgBig>(g, batch); break;
-      case 13: launch_mx<h16, false, 1, h16, true, 1, CfgBig>(g, batch); break;
-      case 14: launch_mx<h16, true, 1, h16, false, 1, CfgBig>(g, batch); break;
-      default: launch_mx<h16, true, 1, h16, true, 1, CfgBig>(g, batch); break;
-    }
-  } else if (a.f32 && !a.kc && a.parts == 2 && b.f32 && !b.kc && b.parts == 2) {
+  if (a.f32 && !a.kc && a.parts == 2 && b.f32 && !b.kc && b.parts == 2) {
     // Gram of the stacked bases
     if (wide) launch_mx<float, false, 2, float, false, 2, CfgTall>(g, batch);
     else launch_mx<float, false, 2, float, false, 2, Cfg128>(g, batch);
@@ -529,13 +510,6 @@ void convert(i64 rows, i64 cols, const Tin *src, i64 lds, Tout *dst, i64 ldd) {
   if (rows <= 0 || cols <= 0) return;
   hipLaunchKernelGGL((convert_kernel<Tout, Tin>), dim3(grid_of(rows * cols)), dim3(256), 0,
                      ctx().stream, rows, cols, src, lds, dst, ldd);
-  LAUNCH_CHECK();
-}
-
-void transpose16(i64 rows, i64 cols, const uint16_t *src, i64 lds, uint16_t *dst, i64 ldd) {
-  if (rows <= 0 || cols <= 0) return;
-  hipLaunchKernelGGL(transpose16_kernel, dim3((unsigned)((rows + 63) / 64), (unsigned)((cols + 63) / 64)),
-                     dim3(256), 0, ctx().stream, rows, cols, src, lds, dst, ldd);
   LAUNCH_CHECK();
 }
 

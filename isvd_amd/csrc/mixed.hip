@@ -2,8 +2,6 @@
 // cited at each function).
 #include "mixed.hpp"
 
-#include <cstdlib>
-
 #include "la.hpp"
 #include "mx.hpp"
 
@@ -41,27 +39,13 @@ void gaussian_omega_b(i64 rows, i64 cols, uint16_t *omegat, i64 ld, unsigned lon
   f32_to_bf16(rows * cols, tmp.get(), omegat);
 }
 
-// sketchBlockRow (src/lib/libisvd/core/stage/@x@_sketch.h:136-139) in BF16
+// sketchBlockRow (src/lib/libisvd/core/stage/@x@_sketch.h:136-139) in BF16: a plain BF16
+// product with FP32 accumulation and output
 void stage_sketch_b(const Dims &d, char ordera, const uint16_t *a, i64 lda,
                     const uint16_t *omegat, i64 ldo, float *yst, i64 ldyst) {
-  MxOperand om, am;
-  om.p = omegat; om.ld = ldo; om.f32 = false; om.kc = false;
-  // op(A)^T(kk, j) = A(j, kk): column-major A has j contiguous, row-major A has kk contiguous
-  am.p = a; am.ld = lda; am.f32 = false; am.kc = (ordera == 'R');
-  const char *e = std::getenv("ISVD_MX_SKETCH_T");
-  const int tmode = e ? std::atoi(e) : 0;
-  DBuf<uint16_t> omt, at;
-  if (tmode >= 1) {  // Omega^T to K-contiguous: Omega (n x Nl, ld Nl) as Nl x n col-major
-    omt.alloc(d.n * d.Nl);
-    transpose16(d.Nl, d.n, omegat, ldo, omt.get(), d.n);
-    om.p = omt.get(); om.ld = d.n; om.kc = true;
-  }
-  if (tmode >= 2 && ordera == 'C') {  // A to row-major
-    at.alloc(d.m * d.n);
-    transpose16(d.m, d.n, a, lda, at.get(), d.n);
-    am.p = at.get(); am.ld = d.n; am.kc = true;
-  }
-  gemm_mx(om, am, d.Nl, d.m, d.n, 1.f, 0.f, yst, ldyst);
+  // op(A)^T: a column-major A (m x n) is transposed, a row-major A is already A^T (n x m)
+  gemm_bf16_lib('N', ordera == 'C' ? 'T' : 'N', d.Nl, d.m, d.n, 1.f, omegat, ldo, a, lda, 0.f,
+                yst, ldyst);
 }
 
 // @x@_orthogonalize_gramian.c:81-98, twice

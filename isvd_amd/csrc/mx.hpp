@@ -28,12 +28,14 @@ struct MxOperand {
 void gemm_mx(const MxOperand &a, const MxOperand &b, i64 m, i64 n, i64 k, float alpha,
              float beta, float *C, i64 ldc, i64 batch = 1, i64 sC = 0, bool upper = false);
 
+// C (m x n, FP32, ldc) := alpha op(A) op(B) + beta C for BF16 A, B (column-major, op 'N' / 'T')
+// through hipBLASLt
+void gemm_bf16_lib(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t *A,
+                   i64 lda, const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc);
+
 // dst (rows x cols, ldd) := src (rows x cols, lds), converted
 template <typename Tout, typename Tin>
 void convert(i64 rows, i64 cols, const Tin *src, i64 lds, Tout *dst, i64 ldd);
-
-// dst (cols x rows, ldd) := src (rows x cols, lds)^T, 16-bit elements
-void transpose16(i64 rows, i64 cols, const uint16_t *src, i64 lds, uint16_t *dst, i64 ldd);
 
 // BF16 bits (round to nearest even) of an FP32 array
 void f32_to_bf16(i64 count, const float *src, uint16_t *dst);

@@ -100,17 +100,21 @@ __global__ void sym_affine_kernel(int n, const T *Z, T *out, T a, T b) {
 
 // out[0] = ||A - c I||_F (one workgroup, fixed order)
 template <typename T>
-__global__ __launch_bounds__(256) void frob_shift_kernel(int n, const T *A, T c, T *out) {
-  __shared__ T red[4];
+__global__ __launch_bounds__(1024) void frob_shift_kernel(int n, const T *A, T c, T *out) {
+  __shared__ T red[16];
   T part = 0;
-  for (int e = threadIdx.x; e < n * n; e += 256) {
+  for (int e = threadIdx.x; e < n * n; e += 1024) {
     const T v = A[e] - ((e % n == e / n) ? c : T(0));
     part += v * v;
   }
   for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
   __syncthreads();
-  if (threadIdx.x == 0) out[0] = sqrt(red[0] + red[1] + red[2] + red[3]);
+  if (threadIdx.x == 0) {
+    T t = 0;
+    for (int w = 0; w < 16; ++w) t += red[w];  // fixed order
+    out[0] = sqrt(t);
+  }
 }
 
 // Gauss-Jordan in-place inverse with partial pivoting, n <= 128, matrix in LDS (row-major,
@@ -263,7 +267,8 @@ void kn_matfun(i64 n, const T *Z, T *C, T *Cinv, T *work, int iters1, int iters2
 
 template <typename T>
 void frob_shift(i64 n, const T *A, T c, T *out) {
-  hipLaunchKernelGGL(frob_shift_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)n, A, c, out);
+  hipLaunchKernelGGL(frob_shift_kernel<T>, dim3(1), dim3(1024), 0, ctx().stream, (int)n, A, c,
+                     out);
   LAUNCH_CHECK();
 }
 

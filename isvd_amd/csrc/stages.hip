@@ -175,8 +175,7 @@ __global__ void transpose_small_kernel(int n, const T *A, i64 lda, T *B, i64 ldb
 template <typename T>
 T read_scalar(const T *dptr) {
   T h;
-  HIP_CHECK(hipMemcpyAsync(&h, dptr, sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
-  sync();
+  read_back(&h, dptr, sizeof(T));
   return h;
 }
 
@@ -315,8 +314,7 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
   omatcopy<T>('N', Nl, l, T(1), M, Nl, xb[0] + Nl * l, Nl);
   prep(0);
   T h[2];
-  HIP_CHECK(hipMemcpyAsync(&h[1], scal.get() + 1, sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
-  sync();
+  read_back(&h[1], scal.get() + 1, sizeof(T));
   int cur = 0;
   i64 it;
   double error = -INFINITY;
@@ -352,8 +350,7 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
             Nl * l, 2 * l * l, Nl * l);
     // the next iterate's Bg, D, Z (needed unless this iteration is the last)
     prep(nxt);
-    HIP_CHECK(hipMemcpyAsync(h, scal.get(), 2 * sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
-    sync();
+    read_back(h, scal.get(), 2 * sizeof(T));
     error = (double)h[0];
     if (trace_on())
       std::fprintf(stderr, "isvd KN iter %lld err %.17g rho %.3g (%s)\n", it, error, rho,
@@ -441,8 +438,7 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
       gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, fgc, l, T(1), bp.get(), Nl);
       dot<T>(Nl * l, bp.get(), bp.get(), scal.get() + 2);
       int h_info = 0;
-      HIP_CHECK(hipMemcpyAsync(&h_info, info.get(), sizeof(int), hipMemcpyDeviceToHost,
-                               ctx().stream));
+      read_back(&h_info, info.get(), sizeof(int));
       phit = (double)half_ninv * (double)read_scalar(scal.get() + 2);
       ISVD_CHECK(h_info == 0, "Wen-Yin step matrix is singular (pivot %d)", h_info);
       if (trace_on())
@@ -492,8 +488,7 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
       dot<T>(Nl * l, d2.get(), md2.get(), scal.get() + 1);
     }
     T t[2];
-    HIP_CHECK(hipMemcpyAsync(t, scal.get(), 2 * sizeof(T), hipMemcpyDeviceToHost, ctx().stream));
-    sync();
+    read_back(t, scal.get(), 2 * sizeof(T));
     taug = std::fabs((double)t[0] / (double)t[1]);
     if (taug < args.taumin) taug = args.taumin;
     if (taug > args.taumax) taug = args.taumax;
