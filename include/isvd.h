@@ -321,6 +321,50 @@ void isvd_bPostprocessGramian(const isvd_Param param, const float *argv, const i
                               const isvd_int_t ldut, float *vt, const isvd_int_t ldvt,
                               const mpi_int_t ut_root, const mpi_int_t vt_root);
 
+/* =========================================================================================
+ * FP8 sketch, the "e" family (no counterpart in the reference).  A is held as OCP e4m3 codes
+ * (e4m3fn: bias 7, largest finite value 448, 0x7F / 0xFF NaN) with one FP32 scale per
+ * 128 x 128 block:  A[i, j] = e4m3(a8[i * lda8 + j]) * scale[(i / 128) * ldscale + j / 128].
+ *   a8      row-major m x n (the n index contiguous), lda8 >= n, lda8 % 16 == 0, 16-byte
+ *           aligned;
+ *   scale   ceil(m / 128) x ceil(n / 128), row-major, ldscale >= ceil(n / 128); every scale
+ *           a power of two in [2^-127, 2^127] (applied on the matrix cores as an exact E8M0
+ *           exponent; any other value fails the call);
+ *   omega   Omega (n x Nl) as unscaled e4m3 codes, column c at omega + c * ldomega (the n
+ *           index contiguous), ldomega >= n, ldomega % 16 == 0.
+ * The seeded forms draw Omega from the FP32 stream of isvd_sIsvd, rounded to e4m3.
+ * isvd_eIsvd sketches from (a8, scale) on FP8 matrix cores (FP32 accumulation) and runs the
+ * remaining stages as isvd_bIsvd on the BF16 copy a (orthogonalization, integration, and the
+ * FP32 Rayleigh-Ritz postprocessing Z^T = Qbar^T A against a).
+ * ========================================================================================= */
+typedef uint8_t isvd_e_val_t;
+/* a8, scale := e4m3 codes and block scales of the FP32 matrix a (ordera 'C' or 'R'): each
+ * scale is the smallest power of two s with max |A_block| <= 448 s (1 for a zero block), each
+ * code A / s rounded to nearest even.  Codes of columns n .. roundup(n, 16) - 1 are zero. */
+void isvd_eQuantize(const char ordera, const isvd_int_t m, const isvd_int_t n, const float *a,
+                    const isvd_int_t lda, isvd_e_val_t *a8, const isvd_int_t lda8, float *scale,
+                    const isvd_int_t ldscale);
+void isvd_eSketchGaussianProjection(const isvd_Param param, const float *argv,
+                                    const isvd_int_t argc, float *retv, const isvd_int_t retc,
+                                    const char dista, const isvd_e_val_t *a8,
+                                    const isvd_int_t lda8, const float *scale,
+                                    const isvd_int_t ldscale, float *yst,
+                                    const isvd_int_t ldyst, const isvd_int_t seed,
+                                    const mpi_int_t mpi_root);
+void isvd_eSketchGaussianProjectionOmega(const isvd_Param param, const char dista,
+                                         const isvd_e_val_t *a8, const isvd_int_t lda8,
+                                         const float *scale, const isvd_int_t ldscale,
+                                         const isvd_e_val_t *omega, const isvd_int_t ldomega,
+                                         float *yst, const isvd_int_t ldyst);
+void isvd_eIsvd(const isvd_Param param, const char *alg_s, const char *alg_o, const char *alg_i,
+                const char *alg_p, const float *argv[4], const isvd_int_t argc[4],
+                float *retv[4], const isvd_int_t retc[4], double time[4], FILE *stream,
+                const char dista, const isvd_e_val_t *a8, const isvd_int_t lda8,
+                const float *scale, const isvd_int_t ldscale, const char ordera,
+                const isvd_b_val_t *a, const isvd_int_t lda, float *s, float *ut,
+                const isvd_int_t ldut, float *vt, const isvd_int_t ldvt, const isvd_int_t seed,
+                const mpi_int_t ut_root, const mpi_int_t vt_root);
+
 #if defined(__cplusplus)
 }
 #endif

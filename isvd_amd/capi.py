@@ -6,7 +6,9 @@ addresses: a numpy array (host memory, staged through HBM by the library) or a d
 address (e.g. ``torch.Tensor.data_ptr()`` of a CUDA tensor, used in place).
 
 The "b" family (mixed precision) takes A and Omega as BF16 bit patterns (numpy uint16 or
-torch.bfloat16) and every other array as FP32.
+torch.bfloat16) and every other array as FP32.  The "e" family takes A as OCP e4m3 codes
+(uint8, row-major, lda8 % 16 == 0) with FP32 power-of-two scales per 128 x 128 block, and a
+BF16 copy of A for the postprocessing.
 
 Storage conventions (all column-major, as in the reference):
   a      : A, m x n with lda >= m (ordera 'C'), or the row-major m x n A with lda >= n ('R')
@@ -97,6 +99,15 @@ def load(path=None):
         _sig(getattr(lib, f"isvd_{x}PostprocessSymmetric_gpu"), post)
         _sig(getattr(lib, f"isvd_{x}IsvdBackward"),
              [Param, ctypes.c_char_p, CH, CH, VP, I, VP, I, VP, VP, I, VP, I, VP, I])
+    # e family: FP8 (e4m3) sketch
+    _sig(lib.isvd_eQuantize, [CH, I, I, VP, I, VP, I, VP, I])
+    _sig(lib.isvd_eSketchGaussianProjection,
+         [Param, VP, I, VP, I, CH, VP, I, VP, I, VP, I, I, mpi_int_t])
+    _sig(lib.isvd_eSketchGaussianProjectionOmega, [Param, CH, VP, I, VP, I, VP, I, VP, I])
+    _sig(lib.isvd_eIsvd,
+         [Param, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
+          VP, VP, VP, VP, VP, VP, CH, VP, I, VP, I, CH, VP, I, VP, VP, I, VP, I, I,
+          mpi_int_t, mpi_int_t])
     _lib = lib
     return lib
 

@@ -11,6 +11,7 @@
 
 #include "../../include/isvd.h"
 #include "la.hpp"
+#include "fp8.hpp"
 #include "mixed.hpp"
 #include "stages.hpp"
 
@@ -478,6 +479,14 @@ static const char *alg_name(char kind, const char *alg, const char *prec) {
   return s.c_str();
 }
 
+// FP8 sketch input of the e family: A as e4m3 codes with per-128 x 128 block scales
+struct E8Input {
+  const uint8_t *a8;
+  i64 lda8;
+  const float *scale;
+  i64 lds;
+};
+
 template <class P>
 static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *alg_o,
                       const char *alg_i, const char *alg_p, const typename P::T *argv[4],
@@ -486,7 +495,7 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
                       const typename P::TA *a, i64 lda, const typename P::TA *omegat, i64 ldo,
                       typename P::T *s, typename P::T *ut, i64 ldut, typename P::T *vt,
                       i64 ldvt, bool use_seed, unsigned long long seed, int ut_root,
-                      int vt_root) {
+                      int vt_root, const E8Input *e8 = nullptr) {
   using T = typename P::T;
   using TA = typename P::TA;
   const char *as = arg2str("ALG_S", alg_s, kOptsS, 2);
@@ -515,11 +524,12 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   if (!dista_ || !ordera_) return;
   check_param(param);
   const Dims d = dims_of(param);
+  const char *prec_name = e8 ? "Mixed Precision (FP8 e4m3 sketch, BF16 A)" : P::name;
   if (stream) {
-    std::fprintf(stream, "Using %s.\n", alg_name('S', as, P::name));
-    std::fprintf(stream, "Using %s.\n", alg_name('O', ao, P::name));
-    std::fprintf(stream, "Using %s.\n", alg_name('I', ai, P::name));
-    std::fprintf(stream, "Using %s.\n\n", alg_name('P', ap, P::name));
+    std::fprintf(stream, "Using %s.\n", alg_name('S', as, prec_name));
+    std::fprintf(stream, "Using %s.\n", alg_name('O', ao, prec_name));
+    std::fprintf(stream, "Using %s.\n", alg_name('I', ai, prec_name));
+    std::fprintf(stream, "Using %s.\n\n", alg_name('P', ap, prec_name));
   }
   // argument checks of every stage, before any device work
   ISVD_CHECK(lda >= (ordera_ == 'C' ? d.m : d.n), "lda (%lld) too small", lda);
@@ -535,6 +545,13 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   if (!use_seed) ISVD_CHECK(ldo >= d.Nl, "ldomegat (%lld) must be >= Nl", ldo);
   if (P::mixed) ISVD_CHECK(!hr && !sy, "HR integration and SY postprocessing have no mixed-precision form");
   if (sy) ISVD_CHECK(d.m == d.n, "symmetric postprocessing needs m == n");
+  if (e8) {
+    ISVD_CHECK(e8->lda8 >= d.n && e8->lda8 % 16 == 0,
+               "lda8 (%lld) must be >= n and a multiple of 16", e8->lda8);
+    ISVD_CHECK(e8->lds >= (d.n + kF8Block - 1) / kF8Block,
+               "ldscale (%lld) must be >= ceil(n / 128)", e8->lds);
+    ISVD_CHECK(e8->a8 != nullptr && e8->scale != nullptr, "a8 and scale must not be NULL");
+  }
   KnArgs ka;
   WyArgs wa;
   if (argc_i > 0 && !hr) {
@@ -570,19 +587,35 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   // sketching (@x@_isvd.c:243-247)
   stage_msg("Sketching ..................... ");
   HIP_CHECK(hipEventRecord(ev[0], ctx().stream));
-  const TA *omd = omegat;
-  Arg<TA> O(use_seed ? nullptr : omegat, use_seed ? 0 : span(d.Nl, d.n, ldo), true, false);
-  if (use_seed) {
-    om.alloc(d.Nl * d.n);
-    if constexpr (P::mixed) gaussian_omega_b(d.Nl, d.n, om.get(), d.Nl, seed);
-    else gaussian_omega<T>(d.Nl, d.n, om.get(), d.Nl, seed);
-    omd = om.get();
-    ldo = d.Nl;
-  } else {
-    omd = O.dev;
+  bool sketched = false;
+  if constexpr (P::mixed) {
+    if (e8) {
+      // e4m3 A (row-major) and its block scales; Omega drawn as e4m3 with K contiguous
+      const i64 nbi = (d.m + kF8Block - 1) / kF8Block, nbj = (d.n + kF8Block - 1) / kF8Block;
+      Arg<uint8_t> A8(e8->a8, span(d.n, d.m, e8->lda8), true, false);
+      Arg<float> SC(e8->scale, span(nbj, nbi, e8->lds), true, false);
+      const i64 ldo8 = (d.n + 15) / 16 * 16;
+      DBuf<uint8_t> om8(ldo8 * d.Nl);
+      gaussian_omega_e(d.n, d.Nl, om8.get(), ldo8, seed);
+      stage_sketch_e(d, A8.dev, e8->lda8, SC.dev, e8->lds, om8.get(), ldo8, yst.get(), d.Nl);
+      sketched = true;
+    }
   }
-  if constexpr (P::mixed) stage_sketch_b(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
-  else stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
+  if (!sketched) {
+    const TA *omd = omegat;
+    Arg<TA> O(use_seed ? nullptr : omegat, use_seed ? 0 : span(d.Nl, d.n, ldo), true, false);
+    if (use_seed) {
+      om.alloc(d.Nl * d.n);
+      if constexpr (P::mixed) gaussian_omega_b(d.Nl, d.n, om.get(), d.Nl, seed);
+      else gaussian_omega<T>(d.Nl, d.n, om.get(), d.Nl, seed);
+      omd = om.get();
+      ldo = d.Nl;
+    } else {
+      omd = O.dev;
+    }
+    if constexpr (P::mixed) stage_sketch_b(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
+    else stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
+  }
   HIP_CHECK(hipEventRecord(ev[1], ctx().stream));
   stage_msg("done.\nOrthogonalizing ............... ");
   if constexpr (P::mixed) {
@@ -677,6 +710,62 @@ static void backward_impl(const isvd_Param &param, const char *alg_p, char dista
   stage_isvd_backward<T>(d, ap[0] == 'T', ordera_, A.dev, lda, Q.dev, ldqt, gs_d, gu_d, ldgut,
                          gv_d, ldgvt, DA.dev, ldda);
   DA.finish();
+  sync();
+}
+
+// e family: the FP8 (e4m3) sketch (no counterpart in the reference)
+static void sketch_e_impl(const isvd_Param &param, char dista, const uint8_t *a8, i64 lda8,
+                          const float *scale, i64 lds, const uint8_t *omega8, i64 ldo,
+                          float *yst, i64 ldyst, bool use_seed, unsigned long long seed) {
+  const char dista_ = arg2char("DISTA", dista, "CR");
+  if (!dista_) return;
+  check_param(param);
+  const Dims d = dims_of(param);
+  if (dista_ == 'C')
+    ISVD_CHECK(ldyst == d.Nl, "ldyst (%lld) must equal Nl (%lld)", ldyst, d.Nl);
+  else
+    ISVD_CHECK(ldyst >= d.Nl, "ldyst (%lld) must be >= Nl (%lld)", ldyst, d.Nl);
+  ISVD_CHECK(lda8 >= d.n && lda8 % 16 == 0, "lda8 (%lld) must be >= n and a multiple of 16",
+             lda8);
+  const i64 nbi = (d.m + kF8Block - 1) / kF8Block, nbj = (d.n + kF8Block - 1) / kF8Block;
+  ISVD_CHECK(lds >= nbj, "ldscale (%lld) must be >= ceil(n / 128) (%lld)", lds, nbj);
+  if (!use_seed)
+    ISVD_CHECK(ldo >= d.n && ldo % 16 == 0,
+               "ldomega (%lld) must be >= n and a multiple of 16", ldo);
+  ISVD_CHECK(a8 != nullptr && scale != nullptr && yst != nullptr, "NULL argument");
+  ensure_init();
+  Arg<uint8_t> A8(a8, span(d.n, d.m, lda8), true, false);
+  Arg<float> SC(scale, span(nbj, nbi, lds), true, false);
+  Arg<float> Y(yst, span(d.Nl, d.m, ldyst), false, true);
+  Arg<uint8_t> O(use_seed ? nullptr : omega8, use_seed ? 0 : span(d.n, d.Nl, ldo), true, false);
+  DBuf<uint8_t> om;
+  const uint8_t *omd = O.dev;
+  if (use_seed) {
+    ldo = (d.n + 15) / 16 * 16;
+    om.alloc(ldo * d.Nl);
+    gaussian_omega_e(d.n, d.Nl, om.get(), ldo, seed);
+    omd = om.get();
+  }
+  stage_sketch_e(d, A8.dev, lda8, SC.dev, lds, omd, ldo, Y.dev, ldyst);
+  Y.finish();
+  sync();
+}
+
+static void quantize_impl(char ordera, i64 m, i64 n, const float *a, i64 lda, uint8_t *a8,
+                          i64 lda8, float *scale, i64 lds) {
+  const char ordera_ = arg2char("ORDERA", ordera, "CR");
+  if (!ordera_) return;
+  ISVD_CHECK(m > 0 && n > 0, "invalid sizes (m=%lld n=%lld)", m, n);
+  ISVD_CHECK(a != nullptr && a8 != nullptr && scale != nullptr, "NULL argument");
+  ensure_init();
+  const i64 nbi = (m + kF8Block - 1) / kF8Block, nbj = (n + kF8Block - 1) / kF8Block;
+  ISVD_CHECK(lds >= nbj, "ldscale (%lld) must be >= ceil(n / 128) (%lld)", lds, nbj);
+  Arg<float> A(a, ordera_ == 'C' ? span(m, n, lda) : span(n, m, lda), true, false);
+  Arg<uint8_t> A8(a8, span(n, m, lda8), false, true);
+  Arg<float> SC(scale, span(nbj, nbi, lds), false, true);
+  quantize_e4m3(ordera_, m, n, A.dev, lda, A8.dev, lda8, SC.dev, lds);
+  A8.finish();
+  SC.finish();
   sync();
 }
 
@@ -938,5 +1027,52 @@ ISVD_DEFINE_HR_SY(s, PrecS, isvd_s_val_t)
 ISVD_DEFINE_HR_SY(d, PrecD, isvd_d_val_t)
 ISVD_DEFINE_BACKWARD(s, PrecS, isvd_s_val_t)
 ISVD_DEFINE_BACKWARD(d, PrecD, isvd_d_val_t)
+
+void isvd_eQuantize(const char ordera, const isvd_int_t m, const isvd_int_t n, const float *a,
+                    const isvd_int_t lda, isvd_e_val_t *a8, const isvd_int_t lda8, float *scale,
+                    const isvd_int_t ldscale) {
+  guarded([&] { quantize_impl(ordera, m, n, a, lda, a8, lda8, scale, ldscale); });
+}
+
+void isvd_eSketchGaussianProjection(const isvd_Param param, const float *argv,
+                                    const isvd_int_t argc, float *retv, const isvd_int_t retc,
+                                    const char dista, const isvd_e_val_t *a8,
+                                    const isvd_int_t lda8, const float *scale,
+                                    const isvd_int_t ldscale, float *yst,
+                                    const isvd_int_t ldyst, const isvd_int_t seed,
+                                    const mpi_int_t mpi_root) {
+  (void)argv; (void)retv; (void)retc; (void)mpi_root;
+  if (argc < 0) return;
+  guarded([&] {
+    sketch_e_impl(param, dista, a8, lda8, scale, ldscale, nullptr, 0, yst, ldyst, true,
+                  (unsigned long long)(long long)seed);
+  });
+}
+
+void isvd_eSketchGaussianProjectionOmega(const isvd_Param param, const char dista,
+                                         const isvd_e_val_t *a8, const isvd_int_t lda8,
+                                         const float *scale, const isvd_int_t ldscale,
+                                         const isvd_e_val_t *omega, const isvd_int_t ldomega,
+                                         float *yst, const isvd_int_t ldyst) {
+  guarded([&] {
+    sketch_e_impl(param, dista, a8, lda8, scale, ldscale, omega, ldomega, yst, ldyst, false, 0);
+  });
+}
+
+void isvd_eIsvd(const isvd_Param param, const char *alg_s, const char *alg_o, const char *alg_i,
+                const char *alg_p, const float *argv[4], const isvd_int_t argc[4],
+                float *retv[4], const isvd_int_t retc[4], double time[4], FILE *stream,
+                const char dista, const isvd_e_val_t *a8, const isvd_int_t lda8,
+                const float *scale, const isvd_int_t ldscale, const char ordera,
+                const isvd_b_val_t *a, const isvd_int_t lda, float *s, float *ut,
+                const isvd_int_t ldut, float *vt, const isvd_int_t ldvt, const isvd_int_t seed,
+                const mpi_int_t ut_root, const mpi_int_t vt_root) {
+  guarded([&] {
+    const E8Input e8{a8, lda8, scale, ldscale};
+    isvd_impl<PrecB>(param, alg_s, alg_o, alg_i, alg_p, argv, argc, retv, retc, time, stream,
+                     dista, ordera, a, lda, nullptr, 0, s, ut, ldut, vt, ldvt, true,
+                     (unsigned long long)(long long)seed, ut_root, vt_root, &e8);
+  });
+}
 
 }  // extern "C"

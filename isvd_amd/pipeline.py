@@ -191,3 +191,69 @@ def isvd_backward(param, a, qbar, gs=None, gu=None, gv=None, alg_p="TS", ordera=
         capi.addr(gv), k, capi.addr(da), _a_ld(a, ordera))
     _check("isvd_backward")
     return da
+
+
+# ---- e family: FP8 (OCP e4m3) sketch ---------------------------------------------------------
+
+def _u8_empty_like_kind(ref, shape):
+    if _is_torch(ref):
+        return _torch().empty(shape, dtype=_torch().uint8, device=ref.device)
+    return np.empty(shape, dtype=np.uint8)
+
+
+def quantize_e4m3(a, ordera="R"):
+    """(a8, scale) of an FP32 matrix (isvd_eQuantize).  ``a`` is row-major m x n (ordera 'R')
+    or the row-major n x m array A^T ('C').  a8: m x lda8 uint8 codes, lda8 = roundup(n, 16);
+    scale: ceil(m/128) x ceil(n/128) FP32 powers of two."""
+    lib = capi.load()
+    _ready(a)
+    m, n = (a.shape[0], a.shape[1]) if ordera == "R" else (a.shape[1], a.shape[0])
+    lda8 = (n + 15) // 16 * 16
+    nbi, nbj = (m + 127) // 128, (n + 127) // 128
+    a8 = _u8_empty_like_kind(a, (m, lda8))
+    scale = _empty_like_kind(a, (nbi, nbj), np.float32)
+    lib.isvd_eQuantize(capi._chr(ordera), m, n, capi.addr(a), a.shape[1], capi.addr(a8), lda8,
+                       capi.addr(scale), nbj)
+    _check("quantize_e4m3")
+    return a8, scale
+
+
+def sketch_e(param, a8, scale, omega8=None, seed=None, dista="R"):
+    """Y (m x Nl, FP32) = A Omega from the e4m3 A (a8 m x lda8, scale per 128 x 128 block).
+    ``omega8`` (Nl x ldo uint8: row c holds column c of Omega, n codes) or ``seed``."""
+    lib = capi.load()
+    _ready(a8, scale, omega8)
+    m, Nl = param.nrow, param.dim_sketch_total
+    y = _empty_like_kind(a8, (m, Nl), np.float32)
+    if omega8 is not None:
+        lib.isvd_eSketchGaussianProjectionOmega(
+            param, capi._chr(dista), capi.addr(a8), a8.shape[1], capi.addr(scale),
+            scale.shape[1], capi.addr(omega8), omega8.shape[1], capi.addr(y), Nl)
+    else:
+        lib.isvd_eSketchGaussianProjection(
+            param, None, 0, None, 0, capi._chr(dista), capi.addr(a8), a8.shape[1],
+            capi.addr(scale), scale.shape[1], capi.addr(y), Nl, int(seed or 0), 0)
+    _check("sketch_e")
+    return y
+
+
+def isvd_e(param, a8, scale, a_bf16, seed=0, algs=("GP", "GR", "KN", "GR"), ordera="R",
+           dista="R", argv_i=None, want_u=True, want_v=True):
+    """isvd_eIsvd: FP8 sketch from (a8, scale), the other stages on the BF16 copy ``a_bf16``
+    (laid out as for ``isvd``).  Returns dict(s, u, v, iter, err, time), FP32."""
+    lib = capi.load()
+    _ready(a8, scale, a_bf16)
+    l = param.dim_sketch
+    s = _empty_like_kind(a8, (l,), np.float32)
+    u = _empty_like_kind(a8, (param.nrow, l), np.float32) if want_u else None
+    v = _empty_like_kind(a8, (param.ncol, l), np.float32) if want_v else None
+    pack = capi.ArgPack(ctypes.c_float, [None, None, list(argv_i) if argv_i else None, None])
+    lib.isvd_eIsvd(param, *(al.encode() for al in algs), pack.argv, pack.argc, pack.retv,
+                   pack.retc, pack.time, None, capi._chr(dista), capi.addr(a8), a8.shape[1],
+                   capi.addr(scale), scale.shape[1], capi._chr(ordera), capi.addr(a_bf16),
+                   _a_ld(a_bf16, ordera), capi.addr(s), capi.addr(u), l, capi.addr(v), l,
+                   int(seed), 0 if want_u else -2, 0 if want_v else -2)
+    _check("isvd_e")
+    it, err = pack.ret(2)[:2]
+    return {"s": s, "u": u, "v": v, "iter": int(it), "err": float(err),
+            "time": list(pack.time)}
