@@ -149,6 +149,89 @@ def mixed_case(torch, capi, pipeline, m, n, k, p, N, r, decay, steps, warmup):
     return out
 
 
+PEAK_FP8_MFMA = 5.0e15  # MI355X dense FP8 matrix peak (FLOP/s)
+
+
+def fp8_case(torch, capi, pipeline, steps, warmup, m=262144, n=32768, k=64, p=64, N=48):
+    """A = U diag(0.97^i) V^T + 1e-4 N(0,1) (rank-1024 planted part) held as OCP e4m3 with
+    FP32 power-of-two scales per 128 x 128 block (isvd_eQuantize) plus a BF16 copy; the e4m3
+    sketch on block-scaled FP8 matrix cores, then GR (twice, FP32) / KN / TS with the FP32
+    Rayleigh-Ritz against the BF16 copy (isvd_eIsvd), timed.  Reference: the FP64 driver on
+    the FP64 matrix with the identical Omega (the e4m3 values of the same seeded stream)."""
+    import numpy as np
+    r = 1024
+    l, Nl = k + p, (k + p) * N
+    algs = ("GP", "GR", "KN", "TS")
+    seed = 2024
+    log(f"fp8: generating A {m}x{n} (FP64, rank {r} + 1e-4 noise)")
+    a64, s_true, _ = make_problem(torch, m, n, r, seed=555, noise=1e-4,
+                                  spectrum=0.97 ** np.arange(r, dtype=np.float64))
+    # Omega of the e family: e4m3 of the FP32 stream, read back through Y = I Omega
+    eye = torch.eye(n, device="cuda", dtype=torch.float32)
+    om32 = pipeline.sketch(capi.create_param(n, n, l, 0, N), eye, seed=seed)
+    del eye
+    om64 = om32.to(torch.float8_e4m3fn).double()
+    del om32
+    prm = capi.create_param(m, n, k, p, N)
+    torch.cuda.synchronize()
+    log("fp8: FP64 reference run")
+    t0 = time.perf_counter()
+    rd = pipeline.isvd(prm, a64, omega=om64, algs=algs)
+    torch.cuda.synchronize()
+    sec64 = time.perf_counter() - t0
+    del om64
+    # BF16 copy and e4m3 + block scales from FP32 row chunks (multiples of 128 rows)
+    a_bf = torch.empty(m, n, device="cuda", dtype=torch.bfloat16)
+    a8 = torch.empty(m, (n + 15) // 16 * 16, device="cuda", dtype=torch.uint8)
+    scale = torch.empty((m + 127) // 128, (n + 127) // 128, device="cuda", dtype=torch.float32)
+    chunk = 16384
+    for r0 in range(0, m, chunk):
+        r1 = min(m, r0 + chunk)
+        a32 = a64[r0:r1].float()
+        a_bf[r0:r1].copy_(a32)
+        q8, qs = pipeline.quantize_e4m3(a32)
+        a8[r0:r1].copy_(q8)
+        scale[r0 // 128:(r1 + 127) // 128].copy_(qs)
+        del a32, q8, qs
+    del a64
+    torch.cuda.empty_cache()
+    torch.cuda.synchronize()
+    for _ in range(warmup):
+        re = pipeline.isvd_e(prm, a8, scale, a_bf, seed=seed, algs=algs)
+    torch.cuda.synchronize()
+    times, stages = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        re = pipeline.isvd_e(prm, a8, scale, a_bf, seed=seed, algs=algs)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        stages.append(re["time"])
+    sec = float(np.mean(times))
+    st = np.mean(stages, axis=0).tolist()
+    log(f"fp8: {sec:.4f} s, stages {np.round(st, 4).tolist()}, KN iter {re['iter']}; "
+        f"FP64 {sec64:.3f} s")
+    se, sd = re["s"][:k].double(), rd["s"][:k]
+    flop = 2.0 * m * n * Nl
+    out = {"workload": f"A {m}x{n} as e4m3 + FP32 power-of-two scales per 128x128 block "
+                       f"({a8.numel() / 1e9:.2f} GB + {scale.numel() * 4 / 2 ** 20:.1f} MiB) and "
+                       f"a BF16 copy; U diag(0.97^i) V^T (rank {r}) + 1e-4 N(0,1); k={k} p={p} "
+                       f"N={N}; GP/GR/KN/TS; errors vs the FP64 driver on the FP64 matrix with "
+                       f"the same (e4m3-valued) Omega",
+           "seconds": sec, "stage_seconds": st, "kn_iterations": re["iter"],
+           "sketch_tflops": flop / st[0] / 1e12,
+           "sketch_frac_of_fp8_peak": flop / st[0] / PEAK_FP8_MFMA,
+           "fp64_seconds": sec64, "fp64_stage_seconds": rd["time"],
+           "fp64_kn_iterations": rd["iter"], "speedup_vs_fp64": sec64 / sec,
+           "sigma_max_rel_err_vs_fp64": float(((se - sd).abs() / sd).max()),
+           "u_angle_max_vs_fp64": principal_angle_max(torch, re["u"][:, :k].double(),
+                                                      rd["u"][:, :k]),
+           "v_angle_max_vs_fp64": principal_angle_max(torch, re["v"][:, :k].double(),
+                                                      rd["v"][:, :k])}
+    del a8, a_bf, scale, re, rd
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -162,6 +245,8 @@ def main():
     ap.add_argument("--no-wy", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp8", action="store_true")
+    ap.add_argument("--fp8-only", action="store_true", help="run only the FP8 (e4m3) case")
     ap.add_argument("--cpu-rows", type=int, default=8192)
     ap.add_argument("--sketch-only", action="store_true",
                     help="run one sketch GEMM and one 1 GiB device copy (for counter runs)")
@@ -193,6 +278,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"sketch_seconds": t1 - t0, "copy_bytes": 2 ** 30,
                           "check": float(dst[0])}))
+        return
+    if args.fp8_only:
+        print(json.dumps({"fp8": fp8_case(torch, capi, pipeline, args.steps, args.warmup)}))
         return
     r = min(2048, n)
     log(f"generating A {m}x{n} (FP64) and Omega {n}x{Nl} on the GPU")
@@ -272,6 +360,13 @@ def main():
         mixed["speedup_vs_fp64_same_input"] = min(c["speedup_vs_fp64"] for c in mixed["cases"])
         a, _, _ = make_problem(torch, m, n, r, seed=1234)
 
+    fp8 = None
+    if not args.no_fp8:
+        del a
+        torch.cuda.empty_cache()
+        fp8 = fp8_case(torch, capi, pipeline, min(args.steps, 2), min(args.warmup, 1))
+        a, _, _ = make_problem(torch, m, n, r, seed=1234)
+
     cpu = None
     if not args.no_cpu:
         log("CPU baseline (oracle on a row sample)")
@@ -288,7 +383,7 @@ def main():
         "stage_seconds": stage_avg, "kn_iterations": res["iter"],
         "accuracy": {"sigma_max_rel_err_vs_planted": sigma_rel_err, "UtU_minus_I": orth_u,
                      "VtV_minus_I": orth_v, "residual_AV_minus_US_rel": resid},
-        "wen_yin": wy, "mixed": mixed, "roofline": roofline, "cpu_baseline": cpu,
+        "wen_yin": wy, "mixed": mixed, "fp8": fp8, "roofline": roofline, "cpu_baseline": cpu,
     }
     print(json.dumps(out))
 

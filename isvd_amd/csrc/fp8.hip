@@ -1,19 +1,18 @@
 // FP8 (OCP e4m3) sketch on the gfx950 block-scaled matrix cores.
 //
 // Product: Y^T[r, i] = sum_j Omega[j, r] A[i, j] over j < n, with A[i, j] =
-// decode(a8[i, j]) * 2^e(i / 128, j / 128).  One K step is 128 bytes: exactly one
-// v_mfma_scale_f32_16x16x128_f8f6f4 per 16 x 16 output fragment, and exactly one scale block
-// along K, so the block's exponent rides in the instruction's B-operand scale (E8M0) and the
-// accumulation stays in the MFMA's FP32 accumulator with no vector-ALU rescaling.
-//
-// Operand layout of the instruction (checked on gfx950): lane l holds row l % 16 of the
-// 16 x 128 A operand (bytes K = 32 (l / 16) .. +32, ascending) and column l % 16 of the B
-// operand (the same K range); D[4 (l / 16) + q][l % 16] is accumulator q of lane l.  Both
-// operands here are K-contiguous rows (Omega columns, A rows), so a fragment is two 16-byte
-// LDS reads per lane.
+// decode(a8[i, j]) * 2^e(i / 128, j / 128).  The product runs on
+// v_mfma_scale_f32_32x32x64_f8f6f4, one K step of 64 bytes per instruction; a 128-wide scale
+// block spans two K steps, so the block's exponent rides in the instruction's B-operand scale
+// (E8M0) and the accumulation stays in the MFMA's FP32 accumulator with no vector-ALU
+// rescaling.  Operand layouts of the instruction (checked on gfx950): lane l holds row l % 32
+// of each operand, K bytes 32 (l / 32) .. +32 ascending; accumulator register q of lane l is
+// D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32].  Both operands here are K-contiguous rows (Omega
+// columns, A rows), so a fragment is two 16-byte LDS reads per lane.
 #include "fp8.hpp"
 
 #include <mutex>
+#include <utility>
 
 #include "la.hpp"
 
@@ -133,22 +132,6 @@ __global__ void scale_e8m0_kernel(i64 nbi, i64 nbj, const float *__restrict__ sc
 // ---------------------------------------------------------------------------------------------
 // the block-scaled GEMM
 // ---------------------------------------------------------------------------------------------
-template <int BR_, int BI_, int WR_, int WI_>
-struct F8Cfg {
-  static constexpr int BR = BR_, BI = BI_, WR = WR_, WI = WI_;
-  static constexpr int NT = 64 * WR * WI;
-  static constexpr int TR = BR / WR, TI = BI / WI;  // wave tile
-  static constexpr int FR = TR / 16, FI = TI / 16;  // 16 x 16 fragments per wave
-  static constexpr int ROWS = BR + BI;              // LDS rows of 128 bytes per stage
-  static constexpr int STAGE = ROWS * 128;
-  static constexpr int LOADS = ROWS * 8 / NT;       // 16-byte chunks per thread per stage
-  static_assert(TR % 16 == 0 && TI % 16 == 0, "wave tile must be a multiple of 16");
-  static_assert(128 % TI == 0, "a wave's A rows must lie in one scale block");
-  static_assert((ROWS * 8) % NT == 0, "chunks must divide evenly");
-};
-
-using F8Main = F8Cfg<128, 256, 1, 4>;
-
 struct F8Args {
   i64 Nl, m, n;
   const uint8_t *om;  // Omega columns (K contiguous), ld ldo
@@ -161,13 +144,75 @@ struct F8Args {
   i64 ldy;
 };
 
-// byte offset of 16-byte chunk c of LDS row `row` (chunks XOR-swizzled against bank conflicts
-// of the 16-row fragment reads)
-__device__ inline int lds_off(int row, int c) { return row * 128 + ((c ^ ((row >> 1) & 7)) << 4); }
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef __attribute__((address_space(3))) void lds_void_t;
 
-template <class C>
-__global__ __launch_bounds__(C::NT) void f8_sketch_kernel(F8Args g) {
-  extern __shared__ __align__(16) unsigned char smem[];
+// s_waitcnt immediate (gfx9 encoding): at most v vector-memory operations outstanding,
+// expcnt and lgkmcnt unconstrained
+constexpr int waitcnt_vm(int v) { return (v & 15) | (7 << 4) | (15 << 8) | ((v >> 4) << 14); }
+
+// One 16-byte-per-lane fill of LDS from global memory (global_load_lds_dwordx4): lane l's 16
+// bytes land at lds_base + 16 l.  Issued as inline assembly so that the compiler does not
+// track it: its automatic waits would otherwise drain every fill in flight before each LDS
+// read.  The kernels wait on the vector-memory counter themselves before a stage is read.
+// M0 holds the LDS base; one wait state must separate its write from the load.
+// M0 is named as clobbered; no other code in the kernels that use this reads M0 (the
+// compiler's only M0 uses are LDS DMA, messages and relative indexing, none of which occur).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ inline void lds_fill16(const void *src, unsigned lds_base) {
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(src), "s"(lds_base)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
+
+__device__ inline unsigned lds_addr(const unsigned char *p) {
+  return __builtin_amdgcn_readfirstlane(
+      (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char *)p);
+}
+
+// 16-byte chunk swizzle of a 64-byte LDS row: chunk c of row r sits at position
+// c ^ swz(r).  Among rows 0..7, and among the rows of each residue mod 4 in 0..15, the
+// positions differ, so a fragment read (consecutive rows, one chunk each) hits distinct
+// banks whether the LDS serves 8 or 16 lanes of a ds_read_b128 per pass.
+__device__ inline int swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+__device__ inline int lds_off(int row, int c) { return row * 64 + ((c ^ swz(row)) << 4); }
+
+// Block tile 256 (Omega rows) x 256 (A rows), four waves in a 2 x 2 grid, 128 x 128 per wave
+// as 4 x 4 fragments of v_mfma_scale_f32_32x32x64_f8f6f4 (lane l: operand row l % 32, K
+// bytes 32 (l / 32) .. +32; accumulator q: row 8 (q / 4) + 4 (l / 32) + q % 4, column l % 32).
+// One K step is 64 bytes: 512 LDS rows of 64 bytes (32 KB) per stage, S stages in flight.
+// Both operands move from HBM straight into LDS with global_load_lds_dwordx4: a wave
+// instruction fills 1 KB (16 rows) in lane order, so lane l owns row 16 j + l / 4, position
+// l % 4, and fetches the logical chunk (l % 4) ^ swz(row).  Rows past the matrix edge are
+// fetched from the last valid row: they only feed outputs that are not stored.  Each wave
+// issues 8 fills per K step; step ks has landed once 8 (S - 2) of the wave's later fills remain.
+template <int S>
+struct F8Cfg {
+  static constexpr int BR = 256, BI = 256, NT = 256;
+  static constexpr int ROWS = BR + BI;
+  static constexpr int STAGE = ROWS * 64;
+  static constexpr int FILLS = ROWS / 16 / 4;  // per wave per K step
+  static constexpr int LDS = STAGE * S;
+};
+
+template <int S>
+__global__ __launch_bounds__(256) void f8_sketch_kernel(F8Args g) {
+  using C = F8Cfg<S>;
+  // one LDS object per stage: the compiler then sees that a fill of one stage and the
+  // fragment reads of another do not alias, and waits only on what a read needs
+  __shared__ __align__(16) unsigned char st0[C::STAGE], st1[C::STAGE], st2[C::STAGE],
+      st3[S > 3 ? C::STAGE : 16], st4[S > 4 ? C::STAGE : 16];
+  auto stage = [&](auto k) -> unsigned char * {
+    constexpr int kk = decltype(k)::value;
+    if constexpr (kk == 0) return st0;
+    else if constexpr (kk == 1) return st1;
+    else if constexpr (kk == 2) return st2;
+    else if constexpr (kk == 3) return st3;
+    else return st4;
+  };
   const i64 nR = (g.Nl + C::BR - 1) / C::BR, nI = (g.m + C::BI - 1) / C::BI;
   const i64 total = nR * nI;
   i64 pid = blockIdx.x;
@@ -181,126 +226,171 @@ __global__ __launch_bounds__(C::NT) void f8_sketch_kernel(F8Args g) {
     pid = (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
   }
   const i64 r0 = (pid % nR) * C::BR, i0 = (pid / nR) * C::BI;
-  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wr = wave % C::WR, wi = wave / C::WR;
-  const i64 nk = (g.n + 127) / 128;
+  // the wave index is uniform: keep it (and the scale reads that depend on it) scalar
+  const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wr = wave & 1, wi = wave >> 1;
 
-  // ---- global -> register staging of one K step
-  u32x4 stage_regs[C::LOADS];
-  auto load_global = [&](i64 k0) {
+  // block-local operand bases: the offsets below stay under 2^32
+  const uint8_t *om_blk = g.om + r0 * g.ldo;
+  const uint8_t *a_blk = g.a8 + i0 * g.lda;
+  const int rows_om = (int)((g.Nl - r0) < C::BR ? (g.Nl - r0) : C::BR);
+  const int rows_a = (int)((g.m - i0) < C::BI ? (g.m - i0) : C::BI);
+  unsigned off[C::FILLS];
 #pragma unroll
-    for (int u = 0; u < C::LOADS; ++u) {
-      const int idx = t + u * C::NT, row = idx >> 3, c = idx & 7;
+  for (int u = 0; u < C::FILLS; ++u) {
+    const int j = u * 4 + wave;              // fill index within the stage
+    const int row = 16 * j + (lane >> 2);    // LDS row
+    const int c = (lane & 3) ^ swz(row);     // logical chunk fetched by this lane
+    if (row < C::BR) {
+      const int rr = row < rows_om ? row : rows_om - 1;
+      off[u] = (unsigned)(rr * g.ldo + 16 * c);
+    } else {
+      const int ii = (row - C::BR) < rows_a ? (row - C::BR) : rows_a - 1;
+      off[u] = (unsigned)(ii * g.lda + 16 * c);
+    }
+  }
+  auto fill = [&](i64 k0, unsigned char *base) {
+#pragma unroll
+    for (int u = 0; u < C::FILLS; ++u) {
+      const int j = u * 4 + wave;
+      const uint8_t *src = (16 * j < C::BR ? om_blk : a_blk) + k0 + off[u];
+      lds_fill16(src, lds_addr(base + j * 1024));
+    }
+  };
+  // the partial last K step: register path with bounds and byte masks, then LDS stores
+  auto fill_checked = [&](i64 k0, unsigned char *base) {
+    for (int q = t; q < C::ROWS * 4; q += C::NT) {
+      const int row = q >> 2, c = q & 3;
       const i64 kk = k0 + 16 * c;
       const uint8_t *src;
       bool ok;
       if (row < C::BR) {
-        ok = (r0 + row < g.Nl);
-        src = g.om + (r0 + row) * g.ldo + kk;
+        ok = row < rows_om;
+        src = om_blk + (i64)row * g.ldo + kk;
       } else {
-        ok = (i0 + row - C::BR < g.m);
-        src = g.a8 + (i0 + row - C::BR) * g.lda + kk;
+        ok = (row - C::BR) < rows_a;
+        src = a_blk + (i64)(row - C::BR) * g.lda + kk;
       }
       u32x4 v = {0u, 0u, 0u, 0u};
       if (ok && kk < g.n) {
         v = *reinterpret_cast<const u32x4 *>(src);
-        const i64 rem = g.n - kk;  // bytes of this chunk inside the matrix
+        const i64 rem = g.n - kk;
         if (rem < 16) {
-#pragma unroll
           for (int w = 0; w < 4; ++w) {
             const i64 keep = rem - 4 * w;
-            const unsigned mask = keep >= 4 ? 0xFFFFFFFFu : (keep <= 0 ? 0u : ((1u << (8 * keep)) - 1u));
-            v[w] &= mask;
+            v[w] &= keep >= 4 ? 0xFFFFFFFFu : (keep <= 0 ? 0u : ((1u << (8 * keep)) - 1u));
           }
         }
       }
-      stage_regs[u] = v;
-    }
-  };
-  auto store_lds = [&](int s) {
-    unsigned char *base = smem + s * C::STAGE;
-#pragma unroll
-    for (int u = 0; u < C::LOADS; ++u) {
-      const int idx = t + u * C::NT, row = idx >> 3, c = idx & 7;
-      *reinterpret_cast<u32x4 *>(base + lds_off(row, c)) = stage_regs[u];
+      *reinterpret_cast<u32x4 *>(base + lds_off(row, c)) = v;
     }
   };
 
-  f32x4 acc[C::FR][C::FI];
+  f32x16 acc[4][4];
 #pragma unroll
-  for (int a = 0; a < C::FR; ++a)
+  for (int a = 0; a < 4; ++a)
 #pragma unroll
-    for (int b = 0; b < C::FI; ++b) acc[a][b] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int b = 0; b < 4; ++b) acc[a][b] = f32x16{};
 
-  const int frow = lane & 15, fc = 2 * (lane >> 4);
-  const i64 bi = (i0 + wi * C::TI) / 128;
-  const int *e8row = g.e8 + bi * g.nbj;
+  // scale row of this wave's 128 A rows (waves wholly past the edge store nothing; clamp)
+  const i64 nbi = (g.m + 127) / 128;
+  const i64 bi_raw = (i0 + 128 * wi) / 128;
+  // read through the constant address space: a scalar load, outside the vector-memory
+  // counter that the stage waits count
+  const __attribute__((address_space(4))) int *e8row =
+      (const __attribute__((address_space(4))) int *)(g.e8 + (bi_raw < nbi ? bi_raw : nbi - 1) * g.nbj);
+  const int frow = lane & 31, fc = 2 * (lane >> 5);
 
-  load_global(0);
-  store_lds(0);
-  __syncthreads();
-  for (i64 ks = 0; ks < nk; ++ks) {
-    const int s = (int)(ks & 1);
-    if (ks + 1 < nk) load_global((ks + 1) * 128);
-    const unsigned char *sO = smem + s * C::STAGE;
-    const unsigned char *sA = sO + C::BR * 128;
-    const int sc = e8row[ks];
-    i32x8 bf[C::FI];
+  auto compute = [&](const unsigned char *st, i64 ks) {
+    const int sc = e8row[ks >> 1];  // scale blocks are 128 = two K steps wide
+    i32x8 bf[4];
 #pragma unroll
-    for (int b = 0; b < C::FI; ++b) {
-      const int row = wi * C::TI + 16 * b + frow;
-      const u32x4 lo = *reinterpret_cast<const u32x4 *>(sA + lds_off(row, fc));
-      const u32x4 hi = *reinterpret_cast<const u32x4 *>(sA + lds_off(row, fc + 1));
+    for (int b = 0; b < 4; ++b) {
+      const int row = C::BR + 128 * wi + 32 * b + frow;
+      const u32x4 lo = *reinterpret_cast<const u32x4 *>(st + lds_off(row, fc));
+      const u32x4 hi = *reinterpret_cast<const u32x4 *>(st + lds_off(row, fc + 1));
       bf[b] = i32x8{(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3],
                     (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
     }
 #pragma unroll
-    for (int a = 0; a < C::FR; ++a) {
-      const int row = wr * C::TR + 16 * a + frow;
-      const u32x4 lo = *reinterpret_cast<const u32x4 *>(sO + lds_off(row, fc));
-      const u32x4 hi = *reinterpret_cast<const u32x4 *>(sO + lds_off(row, fc + 1));
+    for (int a = 0; a < 4; ++a) {
+      const int row = 128 * wr + 32 * a + frow;
+      const u32x4 lo = *reinterpret_cast<const u32x4 *>(st + lds_off(row, fc));
+      const u32x4 hi = *reinterpret_cast<const u32x4 *>(st + lds_off(row, fc + 1));
       const i32x8 af = i32x8{(int)lo[0], (int)lo[1], (int)lo[2], (int)lo[3],
                              (int)hi[0], (int)hi[1], (int)hi[2], (int)hi[3]};
 #pragma unroll
-      for (int b = 0; b < C::FI; ++b)
-        acc[a][b] = __builtin_amdgcn_mfma_scale_f32_16x16x128_f8f6f4(af, bf[b], acc[a][b], 0, 0,
-                                                                     0, 127, 0, sc);
+      for (int b = 0; b < 4; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(af, bf[b], acc[a][b], 0, 0,
+                                                                    0, 127, 0, sc);
     }
-    if (ks + 1 < nk) store_lds(s ^ 1);
+  };
+
+  // ---- full K steps, S - 1 in flight ahead of the one computed
+  const i64 nkf = g.n / 64;
+  const bool tail = (g.n % 64) != 0;
+  // prologue: steps 0 .. S - 2 into stages 0 .. S - 2
+  [&]<int... P>(std::integer_sequence<int, P...>) {
+    ((P < nkf ? fill(64 * P, stage(std::integral_constant<int, P>{})) : void()), ...);
+  }(std::make_integer_sequence<int, S - 1>{});
+  // step ks uses stage ks % S and refills stage (ks - 1) % S with step ks + S - 1
+  auto step = [&](i64 ks, auto k) {
+    constexpr int kk = decltype(k)::value;
+    if (ks + S - 2 < nkf) __builtin_amdgcn_s_waitcnt(waitcnt_vm(C::FILLS * (S - 2)));
+    else __builtin_amdgcn_s_waitcnt(0);
+    // every wave's fills of step ks are in; stage (ks - 1) % S is free.  A bare s_barrier:
+    // the waits above are exact, a workgroup fence would wait for every fill in flight
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+    if (ks + S - 1 < nkf)
+      fill(64 * (ks + S - 1), stage(std::integral_constant<int, (kk + S - 1) % S>{}));
+    compute(stage(k), ks);
+  };
+  for (i64 kb = 0; kb < nkf; kb += S) {
+    [&]<int... P>(std::integer_sequence<int, P...>) {
+      ((kb + P < nkf ? step(kb + P, std::integral_constant<int, P>{}) : void()), ...);
+    }(std::make_integer_sequence<int, S>{});
+  }
+  if (tail) {
+    __syncthreads();  // every wave is done with the stages
+    fill_checked(64 * nkf, st0);
     __syncthreads();
+    compute(st0, nkf);
   }
 
-  // ---- epilogue: lane holds rows r .. r+3 of one column i
+  // ---- epilogue: accumulator registers 4 q0 .. 4 q0 + 3 are 4 consecutive rows of one column
   const bool vec = (g.ldy % 4) == 0 && ((reinterpret_cast<uintptr_t>(g.y) & 15) == 0);
 #pragma unroll
-  for (int a = 0; a < C::FR; ++a) {
-    const i64 r = r0 + wr * C::TR + 16 * a + 4 * (lane >> 4);
+  for (int a = 0; a < 4; ++a) {
 #pragma unroll
-    for (int b = 0; b < C::FI; ++b) {
-      const i64 i = i0 + wi * C::TI + 16 * b + frow;
-      if (i >= g.m || r >= g.Nl) continue;
-      float *dst = g.y + r + i * g.ldy;
-      if (vec && r + 3 < g.Nl) {
-        *reinterpret_cast<f32x4 *>(dst) = acc[a][b];
-      } else {
-        for (int q = 0; q < 4 && r + q < g.Nl; ++q) dst[q] = acc[a][b][q];
+    for (int b = 0; b < 4; ++b) {
+      const i64 i = i0 + 128 * wi + 32 * b + frow;
+      if (i >= g.m) continue;
+#pragma unroll
+      for (int q0 = 0; q0 < 4; ++q0) {
+        const i64 r = r0 + 128 * wr + 32 * a + 8 * q0 + 4 * (lane >> 5);
+        if (r >= g.Nl) continue;
+        float *dst = g.y + r + i * g.ldy;
+        const f32x4 v = {acc[a][b][4 * q0], acc[a][b][4 * q0 + 1], acc[a][b][4 * q0 + 2],
+                         acc[a][b][4 * q0 + 3]};
+        if (vec && r + 3 < g.Nl) {
+          *reinterpret_cast<f32x4 *>(dst) = v;
+        } else {
+          for (int q = 0; q < 4 && r + q < g.Nl; ++q) dst[q] = v[q];
+        }
       }
     }
   }
 }
 
-template <class C>
+template <int S>
 void launch_f8(const F8Args &g) {
-  auto kern = f8_sketch_kernel<C>;
-  const int lds = 2 * C::STAGE;
-  static std::once_flag once;
-  std::call_once(once, [&] {
-    HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                  lds));
-  });
+  using C = F8Cfg<S>;
   const i64 tiles = ((g.Nl + C::BR - 1) / C::BR) * ((g.m + C::BI - 1) / C::BI);
   ISVD_CHECK(tiles <= 0x7FFFFFFF, "too many tiles");
-  hipLaunchKernelGGL(kern, dim3((unsigned)tiles), dim3(C::NT), lds, ctx().stream, g);
+  hipLaunchKernelGGL(f8_sketch_kernel<S>, dim3((unsigned)tiles), dim3(C::NT), 0, ctx().stream,
+                     g);
   LAUNCH_CHECK();
 }
 
@@ -355,8 +445,10 @@ void stage_sketch_e(const Dims &d, const uint8_t *a8, i64 lda8, const float *sca
   int bad_h = 0;
   read_back(&bad_h, bad.get(), sizeof(int));
   ISVD_CHECK(bad_h == 0, "every block scale must be a power of two in [2^-127, 2^127]");
+  ISVD_CHECK(256 * lda8 < (1ll << 32) && 256 * ldo < (1ll << 32), "n too large (lda8 %lld)", lda8);
   F8Args g{Nl, m, n, omega8, ldo, a8, lda8, e8.get(), nbj, yst, ldyst};
-  launch_f8<F8Main>(g);
+  // three stages: at this tile the time per K step does not change from 3 to 5 stages
+  launch_f8<3>(g);
 }
 
 }  // namespace isvd
