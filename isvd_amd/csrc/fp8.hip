@@ -218,14 +218,20 @@ __global__ __launch_bounds__(256) void f8_sketch_kernel(F8Args g) {
   i64 pid = blockIdx.x;
   if (pid >= total) return;
   {
-    // consecutive tiles on one XCD (blocks are dealt round-robin to the 8 XCDs) share the
-    // A rows: the Omega tile index runs fastest
+    // blocks are dealt round-robin to the 8 XCDs: give each XCD a contiguous range of tiles
     constexpr i64 kXcd = 8;
     const i64 xcd = pid % kXcd, local = pid / kXcd;
     const i64 per = total / kXcd, rem = total % kXcd;
     pid = (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
   }
-  const i64 r0 = (pid % nR) * C::BR, i0 = (pid / nR) * C::BI;
+  // grouped raster: tiles run in groups of kGroup A tiles, the A tile index fastest, so the
+  // blocks resident on one XCD (32) share 8 A tiles and 4 Omega tiles in its L2 instead of
+  // streaming a different Omega tile each
+  constexpr i64 kGroup = 8;
+  const i64 grp = pid / (kGroup * nR), first_i = grp * kGroup;
+  const i64 gsz = (nI - first_i) < kGroup ? (nI - first_i) : kGroup;
+  const i64 within = pid % (kGroup * nR);
+  const i64 r0 = (within / gsz) * C::BR, i0 = (first_i + within % gsz) * C::BI;
   // the wave index is uniform: keep it (and the scale reads that depend on it) scalar
   const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
   const int wr = wave & 1, wi = wave >> 1;
