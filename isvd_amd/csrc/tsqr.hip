@@ -77,20 +77,35 @@ __device__ __forceinline__ void panel_of(const LevelArgs &a, int b, int j, T *&b
   }
 }
 
-// sum over the 16 lanes of a row group (xor 8, 4, 2, 1), for kCPL values at once
+// Cross-lane moves inside a DPP row (16 lanes); 64-bit values move as two 32-bit halves.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the 16 lanes of a row group, every lane receiving the same (bitwise) total: pairs
+// (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror), rows
+// (row_mirror).  Each step adds two partial sums in both orders, which FP addition makes equal.
 template <typename T>
-__device__ __forceinline__ void group16_sum(T (&v)[kCPL]) {
-#pragma unroll
-  for (int off = 8; off > 0; off >>= 1)
-#pragma unroll
-    for (int t = 0; t < kCPL; ++t) v[t] += __shfl_xor(v[t], off, 16);
+__device__ __forceinline__ T group16_sum1(T v) {
+  v += dpp_mov<0xB1>(v);
+  v += dpp_mov<0x4E>(v);
+  v += dpp_mov<0x141>(v);
+  v += dpp_mov<0x140>(v);
+  return v;
 }
 
 template <typename T>
-__device__ __forceinline__ T group16_sum1(T v) {
+__device__ __forceinline__ void group16_sum(T (&v)[kCPL]) {
 #pragma unroll
-  for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
-  return v;
+  for (int t = 0; t < kCPL; ++t) v[t] = group16_sum1(v[t]);
 }
 
 template <typename T>
@@ -99,7 +114,8 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
   __shared__ T tau_sh[2];
   const int j_node = blockIdx.x, b = blockIdx.y;
   const int l = a.l, cpw = a.LC / kWaves;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // the wave index is uniform: keep it, and what depends on it, in scalar registers
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int cs = lane >> 4, rg = lane & 15;
   T *base;
   i64 ld;
@@ -221,7 +237,8 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
   __shared__ T vsh[2][kHMax];
   const int j_node = blockIdx.x, b = blockIdx.y;
   const int l = a.l, cpw = a.LC / kWaves;
-  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  // the wave index is uniform: keep it, and what depends on it, in scalar registers
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int cs = lane >> 4, rg = lane & 15;
   T *base;
   i64 ld;
