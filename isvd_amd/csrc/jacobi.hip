@@ -10,6 +10,8 @@
 // A sweep with no rotation above the relative threshold ends the iteration.
 #include "la.hpp"
 
+#include "dpp.hpp"
+
 #include <cfloat>
 
 namespace isvd {
@@ -21,7 +23,11 @@ template <> struct Eps<double> { static constexpr double v = DBL_EPSILON; };
 template <> struct Eps<float> { static constexpr float v = FLT_EPSILON; };
 
 constexpr int kMaxSweeps = 40;
+// 16 waves, 16 lanes per column pair at n = 128 (8 rows per lane): a round is a chain of LDS
+// reads, three reductions and the scalar rotation formula, and 16 waves hide it best (4 and 8
+// waves per matrix were measured slower).
 constexpr int kJacobiThreads = 1024;
+constexpr int kMaxRpl = 8;  // rows per lane: n <= 128 rows over >= 16 lanes per pair
 
 // round-robin (circle method) pairing of np players (np even), round r in [0, np-1)
 __device__ __forceinline__ void rr_pair(int np, int r, int k, int &p, int &q) {
@@ -35,10 +41,10 @@ __device__ __forceinline__ void rr_pair(int np, int r, int k, int &p, int &q) {
   }
 }
 
+// sum over aligned groups of `width` lanes (a power of two <= 64)
 template <typename T>
 __device__ __forceinline__ T group_sum(T v, int width) {
-  for (int off = width >> 1; off > 0; off >>= 1) v += __shfl_xor(v, off, width);
-  return v;
+  return lane_group_sum(v, width);
 }
 
 // mode 0: SVD of A (general n x n).  mode 1: symmetric eigendecomposition of the matrix whose
@@ -49,7 +55,9 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
     T *__restrict__ U, i64 ldu, i64 sU, T *__restrict__ Vwork, T *__restrict__ V, i64 ldv,
     i64 sV, T *__restrict__ shift_out) {
   extern __shared__ unsigned char smem_raw[];
-  T *G = reinterpret_cast<T *>(smem_raw);  // n x n, ld n
+  // n x n, leading dimension n + 1: the four column pairs a wave works on start in
+  // different banks
+  T *G = reinterpret_cast<T *>(smem_raw);
   __shared__ T red[kJacobiThreads / 64];
   __shared__ T nrm[128];
   __shared__ int rank_of[128];
@@ -77,13 +85,16 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
     } else {
       v = Ab[i + (i64)j * lda];
     }
-    G[i + j * n] = v;
+    G[i + j * (n + 1)] = v;
   }
   __syncthreads();
   if (mode == 1) {
     // c = 2 ||A||_F (+1 when A == 0), deterministic block reduction
     T part = 0;
-    for (int e = tid; e < n * n; e += kJacobiThreads) part += G[e] * G[e];
+    for (int e = tid; e < n * n; e += kJacobiThreads) {
+      const T g = G[e % n + (e / n) * (n + 1)];
+      part += g * g;
+    }
     part = group_sum(part, 64);
     if ((tid & 63) == 0) red[tid >> 6] = part;
     __syncthreads();
@@ -95,7 +106,7 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
       shift_sh = c;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += kJacobiThreads) G[i + i * n] += shift_sh;
+    for (int i = tid; i < n; i += kJacobiThreads) G[i + i * (n + 1)] += shift_sh;
     __syncthreads();
   }
   T *Vb = WANT_V ? (Vwork + (i64)b * n * n) : nullptr;
@@ -116,25 +127,34 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
         int p, q;
         rr_pair(np, r, grp, p, q);
         if (p < n && q < n) {
+          // the two columns stay in registers between the dot products and the rotation
+          T gp[kMaxRpl], gq[kMaxRpl];
           T alpha = 0, beta = 0, gamma = 0;
-          for (int i = sub; i < n; i += tpp) {
-            const T gp = G[i + p * n], gq = G[i + q * n];
-            alpha += gp * gp;
-            beta += gq * gq;
-            gamma += gp * gq;
+#pragma unroll
+          for (int u = 0; u < kMaxRpl; ++u) {
+            const int i = sub + u * tpp;
+            gp[u] = (i < n) ? G[i + p * (n + 1)] : T(0);
+            gq[u] = (i < n) ? G[i + q * (n + 1)] : T(0);
+            alpha += gp[u] * gp[u];
+            beta += gq[u] * gq[u];
+            gamma += gp[u] * gq[u];
           }
           alpha = group_sum(alpha, tpp);
           beta = group_sum(beta, tpp);
           gamma = group_sum(gamma, tpp);
-          if (gamma != T(0) && fabs(gamma) > tol * sqrt(alpha * beta)) {
+          // |gamma| > tol sqrt(alpha beta), compared in squares
+          if (gamma != T(0) && gamma * gamma > tol * tol * (alpha * beta)) {
             const T zeta = (beta - alpha) / (T(2) * gamma);
             const T t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
             const T c = T(1) / sqrt(T(1) + t * t);
             const T s = c * t;
-            for (int i = sub; i < n; i += tpp) {
-              const T gp = G[i + p * n], gq = G[i + q * n];
-              G[i + p * n] = c * gp - s * gq;
-              G[i + q * n] = s * gp + c * gq;
+#pragma unroll
+            for (int u = 0; u < kMaxRpl; ++u) {
+              const int i = sub + u * tpp;
+              if (i < n) {
+                G[i + p * (n + 1)] = c * gp[u] - s * gq[u];
+                G[i + q * (n + 1)] = s * gp[u] + c * gq[u];
+              }
             }
             if (WANT_V) {
               for (int i = sub; i < n; i += tpp) {
@@ -156,7 +176,7 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
   // ---- singular values (column norms), ranks for descending order ----
   for (int j = tid / 64; j < n; j += kJacobiThreads / 64) {
     T a = 0;
-    for (int i = tid % 64; i < n; i += 64) a += G[i + j * n] * G[i + j * n];
+    for (int i = tid % 64; i < n; i += 64) a += G[i + j * (n + 1)] * G[i + j * (n + 1)];
     a = group_sum(a, 64);
     if ((tid & 63) == 0) nrm[j] = sqrt(a);
   }
@@ -178,7 +198,7 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
     for (int e = tid; e < n * n; e += kJacobiThreads) {
       const int i = e % n, j = e / n;
       const T sj = nrm[j];
-      Ub[i + (i64)rank_of[j] * ldu] = (sj > T(0)) ? G[e] / sj : T(0);
+      Ub[i + (i64)rank_of[j] * ldu] = (sj > T(0)) ? G[i + j * (n + 1)] / sj : T(0);
     }
   }
   if (WANT_V && V) {
@@ -207,7 +227,7 @@ template <typename T>
 void launch_svd(i64 n, int mode, const T *A, i64 lda, i64 sA, T *s, i64 lds, T *U, i64 ldu,
                 i64 sU, T *V, i64 ldv, i64 sV, i64 batch, T *shift_out) {
   ISVD_CHECK(n >= 1 && n <= 128, "small SVD supports 1 <= n <= 128 (got %lld)", n);
-  const size_t shmem = sizeof(T) * (size_t)n * (size_t)n;
+  const size_t shmem = sizeof(T) * (size_t)n * (size_t)(n + 1);
   DBuf<T> vwork;
   if (V) {
     vwork.alloc(batch * n * n);

@@ -15,6 +15,8 @@
 // Reflectors follow LAPACK dlarfg (beta = -sign(alpha) * ||x||, tau = (beta - alpha) / beta).
 #include "la.hpp"
 
+#include "dpp.hpp"
+
 namespace isvd {
 
 namespace {
@@ -77,29 +79,9 @@ __device__ __forceinline__ void panel_of(const LevelArgs &a, int b, int j, T *&b
   }
 }
 
-// Cross-lane moves inside a DPP row (16 lanes); 64-bit values move as two 32-bit halves.
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double v) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffll), CTRL, 0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, 0xF, 0xF, false);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-template <int CTRL>
-__device__ __forceinline__ float dpp_mov(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
-}
-
-// Sum over the 16 lanes of a row group, every lane receiving the same (bitwise) total: pairs
-// (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1), half rows (row_half_mirror), rows
-// (row_mirror).  Each step adds two partial sums in both orders, which FP addition makes equal.
 template <typename T>
 __device__ __forceinline__ T group16_sum1(T v) {
-  v += dpp_mov<0xB1>(v);
-  v += dpp_mov<0x4E>(v);
-  v += dpp_mov<0x141>(v);
-  v += dpp_mov<0x140>(v);
-  return v;
+  return row16_sum(v);
 }
 
 template <typename T>
