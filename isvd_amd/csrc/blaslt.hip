@@ -1,7 +1,9 @@
-// Plain BF16 x BF16 -> FP32 products through hipBLASLt (the library GEMM).  The mixed
-// sketch Y^T = Omega^T A^T is such a plain product; on this shape hipBLASLt's MFMA kernel is
-// ~3x faster than the hand-written gemm_mx_kernel (14.7 ms vs 46 ms at 4096 x 131072 x 16384),
-// whose fused split-BF16 forms remain in use for the Gram of the bases and the projection.
+// Plain BF16 x BF16 -> FP32 products through hipBLASLt (the library GEMM): the mixed sketch
+// Y^T = Omega^T A^T, and the split-BF16 Gram and projection once their FP32 operands are
+// materialized as BF16 terms (bf16.hip).  On these shapes the library's MFMA kernels ran 2.5-4x
+// faster than a hand-written kernel that split the FP32 operands on the way into LDS (46 ms vs
+// 14.7 ms for the sketch at 4096 x 131072 x 16384; 68.8 ms vs ~33 ms for the Gram at
+// 6144 x 6144 x 262144).
 #include <hipblaslt/hipblaslt.h>
 
 #include "mx.hpp"

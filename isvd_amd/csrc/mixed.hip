@@ -23,11 +23,19 @@ int grid_for(i64 n) {
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
 }
 
-// M := Qs^T Qs (Nl x Nl) from the FP32 bases by split-BF16 x3 products
+// M := Qs^T Qs (Nl x Nl) from the FP32 bases by split-BF16 x3 products: Qs = H + L (two BF16
+// terms), M = H^T H + L^T H + H^T L.  The split is materialized as S = [H^T ; L^T] (2Nl x m)
+// and the two products [H^T H ; L^T H] = S H are one plain BF16 GEMM (hipBLASLt, FP32
+// accumulation); the combination symmetrizes exactly.
 void gram_split(const Dims &d, const float *yst, i64 ldyst, float *m32) {
-  MxOperand q;
-  q.p = yst; q.ld = ldyst; q.f32 = true; q.kc = false; q.parts = 2;
-  gemm_mx(q, q, d.Nl, d.Nl, d.m, 1.f, 0.f, m32, d.Nl, 1, 0, true);
+  const i64 Nl = d.Nl, m = d.m;
+  DBuf<uint16_t> S(2 * Nl * m);
+  split_bf16(Nl, m, 2, yst, ldyst, S.get(), 2 * Nl);
+  DBuf<float> C(2 * Nl * Nl);
+  gemm_bf16_lib('N', 'T', 2 * Nl, Nl, m, 1.f, S.get(), 2 * Nl, S.get(), 2 * Nl, 0.f, C.get(),
+                2 * Nl);
+  S.release();
+  gram_combine(Nl, C.get(), 2 * Nl, m32, Nl);
 }
 
 }  // namespace
@@ -50,12 +58,17 @@ void stage_sketch_b(const Dims &d, char ordera, const uint16_t *a, i64 lda,
 
 // @x@_orthogonalize_gramian.c:81-98, twice
 void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst) {
-  const i64 l = d.l, N = d.N, m = d.m;
-  DBuf<float> w32(l * l * N), u32(l * l * N), ycopy(ldyst * m);
+  const i64 l = d.l, N = d.N, m = d.m, Nl = d.Nl;
+  DBuf<float> w32(l * l * N), u32(l * l * N), other(Nl * m);
   DBuf<double> w64(l * l * N), u64(l * l * N), s64(l * N);
+  // pass 1 reads yst and writes `other`, pass 2 reads `other` and writes yst: no copies
   for (int pass = 0; pass < 2; ++pass) {
+    const float *src = pass == 0 ? yst : other.get();
+    const i64 lds = pass == 0 ? ldyst : Nl;
+    float *dst = pass == 0 ? other.get() : yst;
+    const i64 ldd = pass == 0 ? Nl : ldyst;
     // W_i := Y_i^T Y_i
-    gemm<float>('N', 'T', l, l, m, 1.f, yst, ldyst, yst, ldyst, 0.f, w32.get(), l, N, l, l, l * l);
+    gemm<float>('N', 'T', l, l, m, 1.f, src, lds, src, lds, 0.f, w32.get(), l, N, l, l, l * l);
     convert<double, float>(l, l * N, w32.get(), l, w64.get(), l);
     // W_i = U_i diag(s_i) U_i^T; U_i := U_i diag(s_i)^{-1/2}
     svd_small<double>(l, w64.get(), l, l * l, s64.get(), l, u64.get(), l, l * l, nullptr, 0, 0, N);
@@ -64,10 +77,7 @@ void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst) {
     LAUNCH_CHECK();
     convert<float, double>(l, l * N, u64.get(), l, u32.get(), l);
     // Q_i^T := U_i^T Y_i^T
-    HIP_CHECK(hipMemcpyAsync(ycopy.get(), yst, sizeof(float) * ldyst * m,
-                             hipMemcpyDeviceToDevice, ctx().stream));
-    gemm<float>('T', 'N', l, m, l, 1.f, u32.get(), l, ycopy.get(), ldyst, 0.f, yst, ldyst, N,
-                l * l, l, l);
+    gemm<float>('T', 'N', l, m, l, 1.f, u32.get(), l, src, lds, 0.f, dst, ldd, N, l * l, l, l);
   }
 }
 
@@ -111,11 +121,16 @@ void stage_post_b(const Dims &d, bool tsqr, char ordera, const uint16_t *a, i64 
                   bool want_u, bool want_v) {
   const i64 l = d.l, m = d.m, n = d.n, k = d.k;
   DBuf<float> zt32(l * n);
-  MxOperand qa, ab;
-  qa.p = qt; qa.ld = ldqt; qa.f32 = true; qa.kc = false; qa.parts = 3;
-  // op(A)(kk, j): column-major A is contiguous along kk
-  ab.p = a; ab.ld = lda; ab.f32 = false; ab.kc = (ordera == 'C');
-  gemm_mx(qa, ab, l, n, m, 1.f, 0.f, zt32.get(), l);
+  {
+    // Qbar^T = h0 + h1 + h2 (three BF16 terms, exact to FP32) stacked as 3l x m; one plain
+    // BF16 GEMM against the BF16 A reads A once, then the three l x n blocks are summed
+    DBuf<uint16_t> q3(3 * l * m);
+    split_bf16(l, m, 3, qt, ldqt, q3.get(), 3 * l);
+    DBuf<float> c3(3 * l * n);
+    gemm_bf16_lib('N', ordera == 'C' ? 'N' : 'T', 3 * l, n, m, 1.f, q3.get(), 3 * l, a, lda,
+                  0.f, c3.get(), 3 * l);
+    sum_blocks(l, n, 3, c3.get(), 3 * l, zt32.get(), l);
+  }
   DBuf<double> zt64(l * n), qt64(l * m), s64(l), ut64(want_u ? l * m : 0),
       vt64(want_v ? l * n : 0);
   convert<double, float>(l, n, zt32.get(), l, zt64.get(), l);
