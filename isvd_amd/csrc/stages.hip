@@ -250,20 +250,26 @@ void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst) {
 // The coupled iteration on A = I - R has residual R_k = I - W_k Y_k with
 // ||R_{k+1}|| <= ||R_k||^2 (3 + ||R_k||) / 4; stage 1 starts from ||R_0|| <= rho, stage 2 from
 // ||I - (I + P)/2|| = max |1 - sqrt(1 - x)| / 2 over |x| <= rho <= rho / (2 (1 + sqrt(1 - rho))).
-// Each stage runs until the bound falls below the unit roundoff.
+// Each stage runs until the bound falls below the unit roundoff; false when that takes more
+// than kMaxNs steps (rho too close to 1: the caller then uses the eigendecomposition).
+constexpr int kMaxNs = 20;
 template <typename T>
-void ns_counts(double rho, int *k1, int *k2) {
+bool ns_counts(double rho, int *k1, int *k2) {
   const double eps = (sizeof(T) == 8) ? 0x1p-53 : 0x1p-24;
+  bool ok = true;
   auto steps = [&](double r) {
     int k = 0;
-    while (r > eps && k < 12) {
+    while (r > eps && k < kMaxNs) {
       r = r * r * (3.0 + r) / 4.0;
       ++k;
     }
+    if (r > eps) ok = false;
     return k < 1 ? 1 : k;
   };
+  if (!(rho < 1.0)) return false;
   *k1 = steps(rho);
   *k2 = steps(rho / (2.0 * (1.0 + std::sqrt(1.0 - rho))));
+  return ok;
 }
 
 // [W1 | W2] (2l x l each, ld 2l) := [F; C^-1 / N] and [F; C^-1]
@@ -284,7 +290,7 @@ __global__ void kn_assemble_kernel(int l, const T *F, const T *Ci, T ninv, T *W)
 //   [X | B | Bg] (Nl x 3l) holds X_c, B_c = Qs^T Q_c and Bg_c = M B_c / N;
 //   [D | Dg] = B_c^T [B_c | Bg_c] / N (one product), Z = Dg - D^T D, rho = ||4 Z||_F;
 //   C, C^-1 from Z (Newton-Schulz with step counts chosen from rho, or the eigendecomposition
-//   when rho >= 0.6);  F = C - D C^-1;
+//   when rho is too close to 1 for kMaxNs steps);  F = C - D C^-1;
 //   [X_+ | B_+] = [X_c B_c] [F; C^-1/N] and [B_c Bg_c] [F; C^-1] (one batched product);
 //   then Bg, D, Dg, Z and rho of the next iterate are formed before the host reads err and
 //   the next rho together (the work after the last iteration is discarded).
@@ -322,10 +328,11 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
     const double rho = 4.0 * (double)h[1];
     // C and inv(C) (:193-212): C = (I/2 + (I/4 - Z)^{1/2})^{1/2} = V diag(s) V^T with
     // s = sqrt(1/2 + sqrt(1/4 - eig(Z)))
-    const bool use_eig = !(rho < 0.6);
+    // Newton-Schulz while it converges within kMaxNs steps per stage (rho < ~0.999), else
+    // the eigendecomposition
+    int k1 = 0, k2 = 0;
+    const bool use_eig = !ns_counts<T>(rho, &k1, &k2);
     if (!use_eig) {
-      int k1, k2;
-      ns_counts<T>(rho, &k1, &k2);
       kn_matfun<T>(l, Z, c.get(), cinv.get(), work.get(), k1, k2);
       frob_shift<T>(l, c.get(), T(1), scal.get());  // ||I - C||_F = ||s - 1||_2
     } else {
