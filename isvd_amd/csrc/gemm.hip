@@ -301,6 +301,29 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
     if (want < 1) want = 1;
     ksplit = (int)want;
   }
+  // Balance: with a few rounds of workgroups over the CUs a partial last round idles most of
+  // them.  For long K, take the split count (up to 16) whose rounds are fullest, as long as the
+  // partial sums stay below a quarter of the operand traffic.
+  {
+    constexpr i64 kCUs = 256;
+    const i64 lds_bytes = 2 * (i64)BK * ((BM + 2) + (BN + 2)) * (i64)sizeof(T);
+    const i64 per_cu = (160 * 1024) / lds_bytes < 1 ? 1 : (160 * 1024) / lds_bytes;
+    const i64 slots = kCUs * per_cu;
+    auto fill = [&](i64 ks) {
+      const i64 w = busy * batch * ks;
+      return (double)w / (double)(((w + slots - 1) / slots) * slots);
+    };
+    if (busy * batch * ksplit < 8 * slots && g.k >= 256 * BK) {
+      const double op_bytes = (double)sizeof(T) * (double)(g.m + g.n) * (double)g.k * batch;
+      int best = ksplit;
+      for (int ks = ksplit + 1; ks <= 16; ++ks) {
+        if ((double)sizeof(T) * ks * (double)g.m * g.n * batch > 0.25 * op_bytes) break;
+        if (g.k / ks < 64 * BK) break;
+        if (fill(ks) > fill(best) + 0.02) best = ks;
+      }
+      ksplit = best;
+    }
+  }
   i64 kchunk = (g.k + ksplit - 1) / ksplit;
   kchunk = (kchunk + BK - 1) / BK * BK;
   ksplit = (int)((g.k + kchunk - 1) / kchunk);
