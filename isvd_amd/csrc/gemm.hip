@@ -8,8 +8,6 @@
 // the current tile's MFMAs (one barrier per K tile).
 #include "la.hpp"
 
-#include <cstdlib>
-
 namespace isvd {
 
 typedef double d4 __attribute__((ext_vector_type(4)));
@@ -349,17 +347,6 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
   }
 }
 
-// ISVD_GEMM_TILE selects the large-tile configuration (for tuning sweeps):
-//   0: 128x256x16, 2x4 waves (default)   1: 128x128x16, 2x2 waves
-//   2: 256x128x16, 4x2 waves             3: 128x128x32, 2x2 waves
-int big_tile_cfg() {
-  static const int cfg = [] {
-    const char *e = std::getenv("ISVD_GEMM_TILE");
-    return e ? std::atoi(e) : 0;
-  }();
-  return cfg;
-}
-
 template <typename T, bool TA, bool TB>
 void launch(GemmArgs<T> g, i64 batch) {
   const bool big = (g.m >= 128 && g.n >= 128);
@@ -367,16 +354,13 @@ void launch(GemmArgs<T> g, i64 batch) {
     launch_cfg<T, TA, TB, 64, 64, 16, 2, 2>(g, batch);
     return;
   }
-  if (g.n <= 128 && big_tile_cfg() == 0) {  // a 256-wide tile would be half empty
+  // 128 x 256 tiles (2 x 4 waves of 64 x 64); 128 x 128 when a 256-wide tile would be half
+  // empty.  (256 x 128, 128 x 128 x 32 and 256 x 256 tiles were measured slower on the sketch.)
+  if (g.n <= 128) {
     launch_cfg<T, TA, TB, 128, 128, 16, 2, 2>(g, batch);
     return;
   }
-  switch (big_tile_cfg()) {
-    case 1: launch_cfg<T, TA, TB, 128, 128, 16, 2, 2>(g, batch); break;
-    case 2: launch_cfg<T, TA, TB, 256, 128, 16, 4, 2>(g, batch); break;
-    case 3: launch_cfg<T, TA, TB, 128, 128, 32, 2, 2>(g, batch); break;
-    default: launch_cfg<T, TA, TB, 128, 256, 16, 2, 4>(g, batch); break;
-  }
+  launch_cfg<T, TA, TB, 128, 256, 16, 2, 4>(g, batch);
 }
 
 }  // namespace

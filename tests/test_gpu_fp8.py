@@ -168,3 +168,19 @@ def test_driver_fp8_sketch_vs_fp64_driver(torch_cuda, algs):
     # measured on gfx950: sigma 2.6e-4, angles 3.1e-2 (U) and 7.3e-3 (V); the FP64 driver
     # itself is 1.3e-2 away from the dense SVD's sigma at this spectrum and sketch size
     assert s_err < 1e-3 and ang_u < 0.1 and ang_v < 0.03
+
+
+def test_host_arrays_are_staged_like_device_arrays(torch_cuda):
+    """The C ABI stages host arrays through HBM: the same inputs given as numpy arrays give
+    bit-identical sketches and quantizations."""
+    m, n, l, N = 384, 300, 16, 3
+    rng = np.random.default_rng(77)
+    a = (rng.standard_normal((m, n)) * 3).astype(np.float32)
+    a8_h, sc_h = pipeline.quantize_e4m3(a)
+    a8_d, sc_d = pipeline.quantize_e4m3(f32_dev(torch_cuda, a))
+    assert np.array_equal(a8_h[:, :n], a8_d.cpu().numpy()[:, :n])
+    assert np.array_equal(sc_h, sc_d.cpu().numpy())
+    p = capi.create_param(m, n, l, 0, N)
+    y_h = pipeline.sketch_e(p, a8_h, sc_h, seed=3)
+    y_d = pipeline.sketch_e(p, a8_d, sc_d, seed=3).cpu().numpy()
+    assert isinstance(y_h, np.ndarray) and np.array_equal(y_h, y_d)
