@@ -15,6 +15,7 @@
 #include <utility>
 
 #include "la.hpp"
+#include "lds_dma.hpp"
 
 namespace isvd {
 
@@ -146,32 +147,6 @@ struct F8Args {
 
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __attribute__((address_space(3))) void lds_void_t;
-
-// s_waitcnt immediate (gfx9 encoding): at most v vector-memory operations outstanding,
-// expcnt and lgkmcnt unconstrained
-constexpr int waitcnt_vm(int v) { return (v & 15) | (7 << 4) | (15 << 8) | ((v >> 4) << 14); }
-
-// One 16-byte-per-lane fill of LDS from global memory (global_load_lds_dwordx4): lane l's 16
-// bytes land at lds_base + 16 l.  Issued as inline assembly so that the compiler does not
-// track it: its automatic waits would otherwise drain every fill in flight before each LDS
-// read.  The kernels wait on the vector-memory counter themselves before a stage is read.
-// M0 holds the LDS base; one wait state must separate its write from the load.
-// M0 is named as clobbered; no other code in the kernels that use this reads M0 (the
-// compiler's only M0 uses are LDS DMA, messages and relative indexing, none of which occur).
-#pragma clang diagnostic push
-#pragma clang diagnostic ignored "-Winline-asm"
-__device__ inline void lds_fill16(const void *src, unsigned lds_base) {
-  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
-               :
-               : "v"(src), "s"(lds_base)
-               : "memory", "m0");
-}
-#pragma clang diagnostic pop
-
-__device__ inline unsigned lds_addr(const unsigned char *p) {
-  return __builtin_amdgcn_readfirstlane(
-      (unsigned)(uintptr_t)(const __attribute__((address_space(3))) unsigned char *)p);
-}
 
 // 16-byte chunk swizzle of a 64-byte LDS row: chunk c of row r sits at position
 // c ^ swz(r).  Among rows 0..7, and among the rows of each residue mod 4 in 0..15, the
@@ -346,9 +321,7 @@ __global__ __launch_bounds__(256) void f8_sketch_kernel(F8Args g) {
     else __builtin_amdgcn_s_waitcnt(0);
     // every wave's fills of step ks are in; stage (ks - 1) % S is free.  A bare s_barrier:
     // the waits above are exact, a workgroup fence would wait for every fill in flight
-    asm volatile("" ::: "memory");
-    __builtin_amdgcn_s_barrier();
-    asm volatile("" ::: "memory");
+    bare_barrier();
     if (ks + S - 1 < nkf)
       fill(64 * (ks + S - 1), stage(std::integral_constant<int, (kk + S - 1) % S>{}));
     compute(stage(k), ks);

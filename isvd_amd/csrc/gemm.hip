@@ -6,7 +6,14 @@
 // walk the contiguous dimension of each operand, so both the "N" and the "T" forms load
 // coalesced.  A register-prefetch double buffer overlaps the next tile's global loads with
 // the current tile's MFMAs (one barrier per K tile).
+// The large FP64 products (128 x 256 tiles) run instead on gemm_dl_kernel, which fills LDS
+// straight from global memory three K steps ahead; the register-staged kernel keeps the
+// smaller tiles, FP32, and operands that are not 16-byte aligned.
 #include "la.hpp"
+#include "lds_dma.hpp"
+
+#include <type_traits>
+#include <utility>
 
 namespace isvd {
 
@@ -55,6 +62,86 @@ struct GemmArgs {
   int upper;     // 1: only tiles touching the upper triangle (symmetric result, mirrored after)
 };
 
+// Tile (tm, tn) of this workgroup; false when it has none.  Workgroups are dispatched
+// round-robin over the 8 XCDs (each with its own L2): remap so that each XCD walks a
+// contiguous range of the grouped raster, so that the tiles running together on one XCD share
+// operand panels through that XCD's L2.  A symmetric product enumerates only the tiles
+// touching the upper triangle, column by column.
+template <typename T>
+__device__ inline bool tile_coords(const GemmArgs<T> &g, int BM, int BN, i64 &tm, i64 &tn) {
+  const i64 tiles_m = (g.m + BM - 1) / BM;
+  const i64 tiles_n = (g.n + BN - 1) / BN;
+  auto upper_count = [&](i64 q) {
+    const i64 c = ((q + 1) * BN + BM - 1) / BM;
+    return c < tiles_m ? c : tiles_m;
+  };
+  i64 total = tiles_m * tiles_n;
+  if (g.upper) {
+    total = 0;
+    for (i64 q = 0; q < tiles_n; ++q) total += upper_count(q);
+  }
+  i64 pid = blockIdx.x;
+  if (pid >= total) return false;
+  {
+    constexpr i64 kXcd = 8;
+    const i64 xcd = pid % kXcd, local = pid / kXcd;
+    const i64 per = total / kXcd, rem = total % kXcd;
+    pid = (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
+  }
+  if (g.upper) {
+    tn = 0;
+    while (pid >= upper_count(tn)) pid -= upper_count(tn++);
+    tm = pid;
+  } else {  // grouped raster: group_m tile rows walked column by column
+    const i64 in_group = (i64)g.group_m * tiles_n;
+    const i64 group_id = pid / in_group;
+    const i64 first_m = group_id * g.group_m;
+    const i64 gsz = (tiles_m - first_m) < g.group_m ? (tiles_m - first_m) : g.group_m;
+    tm = first_m + (pid % in_group) % gsz;
+    tn = (pid % in_group) / gsz;
+  }
+  return true;
+}
+
+// Epilogue of one wave: fragment (a, b) holds rows r0 + 16 a + Mfma::row(lane, q) and column
+// c0 + 16 b + lane % 16.  Without split-K, C := alpha acc + beta C; with it, the raw partial
+// goes to the workspace slice of chunk ks (dense m x n).
+template <typename T, int FM, int FN>
+__device__ inline void store_tile(const GemmArgs<T> &g,
+                                  const typename Mfma<T>::acc_t (&acc)[FM][FN], i64 r0, i64 c0,
+                                  int lane, int batch_id, int ks) {
+  const int lr = lane & 15;
+  if (g.ksplit == 1) {
+    T *Cb = g.C + (i64)batch_id * g.sC;
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const i64 r = r0 + a * 16 + Mfma<T>::row(lane, q);
+          const i64 c = c0 + b * 16 + lr;
+          if (r < g.m && c < g.n) {
+            T v = g.alpha * acc[a][b][q];
+            if (g.beta != T(0)) v += g.beta * Cb[r + c * g.ldc];
+            Cb[r + c * g.ldc] = v;
+          }
+        }
+  } else {
+    T *Wb = g.W + ((i64)batch_id * g.ksplit + ks) * g.m * g.n;
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const i64 r = r0 + a * 16 + Mfma<T>::row(lane, q);
+          const i64 c = c0 + b * 16 + lr;
+          if (r < g.m && c < g.n) Wb[r + c * g.m] = acc[a][b][q];
+        }
+  }
+}
+
 template <typename T, bool TA, bool TB, int BM, int BN, int BK, int WGM, int WGN>
 __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs<T> g) {
   constexpr int NT = 64 * WGM * WGN;
@@ -76,43 +163,8 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs<T> g) {
   const int wave = tid >> 6;
   const int wm = wave % WGM, wn = wave / WGM;
 
-  // ---- tile coordinates ----
-  // Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).  Remap so
-  // that each XCD walks a contiguous range of the grouped raster: the tiles running together
-  // on one XCD then share operand panels through that XCD's L2.
-  const i64 tiles_m = (g.m + BM - 1) / BM;
-  const i64 tiles_n = (g.n + BN - 1) / BN;
-  // a symmetric product enumerates only the tiles touching the upper triangle
-  auto upper_count = [&](i64 tn) {
-    const i64 c = ((tn + 1) * BN + BM - 1) / BM;
-    return c < tiles_m ? c : tiles_m;
-  };
-  i64 total = tiles_m * tiles_n;
-  if (g.upper) {
-    total = 0;
-    for (i64 q = 0; q < tiles_n; ++q) total += upper_count(q);
-  }
-  i64 pid = blockIdx.x;
-  if (pid >= total) return;
-  {
-    constexpr i64 kXcd = 8;
-    const i64 xcd = pid % kXcd, local = pid / kXcd;
-    const i64 per = total / kXcd, rem = total % kXcd;
-    pid = (xcd < rem) ? xcd * (per + 1) + local : rem * (per + 1) + (xcd - rem) * per + local;
-  }
   i64 tm, tn;
-  if (g.upper) {  // column by column over the upper tiles
-    tn = 0;
-    while (pid >= upper_count(tn)) pid -= upper_count(tn++);
-    tm = pid;
-  } else {  // grouped raster: group_m tile rows walked column by column
-    const i64 in_group = (i64)g.group_m * tiles_n;
-    const i64 group_id = pid / in_group;
-    const i64 first_m = group_id * g.group_m;
-    const i64 gsz = (tiles_m - first_m) < g.group_m ? (tiles_m - first_m) : g.group_m;
-    tm = first_m + (pid % in_group) % gsz;
-    tn = (pid % in_group) / gsz;
-  }
+  if (!tile_coords(g, BM, BN, tm, tn)) return;
   const i64 m0 = tm * BM, n0 = tn * BN;
 
   const int batch_id = blockIdx.y / g.ksplit;
@@ -199,36 +251,227 @@ __global__ __launch_bounds__(64 * WGM * WGN) void gemm_kernel(GemmArgs<T> g) {
     __syncthreads();
   }
 
-  // ---- epilogue ----
-  if (g.ksplit == 1) {
-    T *Cb = g.C + (i64)batch_id * g.sC;
+  store_tile<T, FM, FN>(g, acc, m0 + wm * WTM, n0 + wn * WTN, lane, batch_id, ks);
+}
+
+// FP64 GEMM with the operands moved from global memory straight into LDS
+// (global_load_lds_dwordx4, no register staging), three K steps in flight.
+//
+// Tile 128 x 256 x 16, eight waves in a 2 x 4 grid of 64 x 64 (4 x 4 fragments of
+// v_mfma_f64_16x16x4f64).  Each operand tile has one of two LDS layouts, chosen by which of
+// its dimensions is contiguous in global memory:
+//   K-contiguous (op(A) = A^T, op(B) = B): one 128-byte row of 16 K values per tile row; the
+//     eight 16-byte chunks of row r sit at position chunk ^ ((r >> 1) & 7), so that the
+//     16 rows of a fragment read fall in distinct banks.  A fill instruction covers 8 rows:
+//     lane l fetches row 8 j + l / 8, chunk (l % 8) ^ swizzle (the LDS position is fixed to
+//     l % 8 by the instruction).
+//   MN-contiguous (op(A) = A, op(B) = B^T): one K row of the tile per 1 KB (A) or 2 KB (B),
+//     padded by 64 bytes so that rows 2 apart start 128 bytes apart in the banks.
+// The MFMA consumes K in pairs: lane l / 16 = s supplies K index k8 + 2 s + h to the
+// instruction h of a pair (h = 0, 1), so a K-contiguous fragment is one 16-byte LDS read
+// serving both instructions, and the two operands agree on the K index of every product.
+// Rows past the matrix edge are fetched from the last valid row (or column pair): they feed
+// only outputs that are not stored.  Requirements (checked by the host): 16-byte aligned
+// operands, even leading dimensions and batch strides, even m (n) when op(A) (op(B)) is
+// MN-contiguous.  A partial last K step goes through registers with bounds checks.
+namespace dl {
+constexpr int BM = 128, BN = 256, BK = 16, NT = 512, S = 3;
+constexpr int PITCH_A = BM * 8 + 64, PITCH_B = BN * 8 + 64;
+constexpr int FILLS = (BM + BN) * BK * 8 / 1024 / (NT / 64);  // per wave per K step
+constexpr int A_FILLS = BM * BK * 8 / 1024;                   // fill instructions for op(A)
+static_assert(A_FILLS % (NT / 64) == 0, "op(A) fills must be whole per-wave slots");
+template <bool KA, bool KB>
+struct Layout {
+  static constexpr int A_BYTES = KA ? BM * BK * 8 : BK * PITCH_A;
+  static constexpr int B_BYTES = KB ? BN * BK * 8 : BK * PITCH_B;
+  static constexpr int STAGE = A_BYTES + B_BYTES;
+  static_assert(S * STAGE <= 160 * 1024, "LDS budget");
+};
+__device__ inline int swz(int r) { return (r >> 1) & 7; }
+}  // namespace dl
+
+typedef double d2 __attribute__((ext_vector_type(2)));
+
+template <bool KA, bool KB>
+__global__ __launch_bounds__(dl::NT) void gemm_dl_kernel(GemmArgs<double> g) {
+  using namespace dl;
+  using L = Layout<KA, KB>;
+  // one LDS object per stage: the compiler then sees that a fill of one stage and the
+  // fragment reads of another do not alias
+  __shared__ __align__(16) unsigned char st0[L::STAGE], st1[L::STAGE], st2[L::STAGE];
+  auto stage = [&](auto k) -> unsigned char * {
+    constexpr int kk = decltype(k)::value;
+    if constexpr (kk == 0) return st0;
+    else if constexpr (kk == 1) return st1;
+    else return st2;
+  };
+  i64 tm, tn;
+  if (!tile_coords(g, BM, BN, tm, tn)) return;
+  const i64 m0 = tm * BM, n0 = tn * BN;
+  const int batch_id = blockIdx.y / g.ksplit;
+  const int ks = blockIdx.y % g.ksplit;
+  const i64 kbeg = (i64)ks * g.kchunk;
+  const i64 kend = (kbeg + g.kchunk < g.k) ? kbeg + g.kchunk : g.k;
+  const double *__restrict__ Ab = g.A + (i64)batch_id * g.sA;
+  const double *__restrict__ Bb = g.B + (i64)batch_id * g.sB;
+  // the wave index is uniform: keep it, and the LDS offsets that depend on it, scalar
+  const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  // fill slot u of this wave is fill instruction f = 8 u + wave of a stage: op(A) for
+  // f < A_FILLS, op(B) after
+  const double *src[FILLS];
+  unsigned loff[FILLS];
 #pragma unroll
-    for (int a = 0; a < FM; ++a)
-#pragma unroll
-      for (int b = 0; b < FN; ++b)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const i64 r = m0 + wm * WTM + a * 16 + Mfma<T>::row(lane, q);
-          const i64 c = n0 + wn * WTN + b * 16 + lr;
-          if (r < g.m && c < g.n) {
-            T v = g.alpha * acc[a][b][q];
-            if (g.beta != T(0)) v += g.beta * Cb[r + c * g.ldc];
-            Cb[r + c * g.ldc] = v;
-          }
-        }
-  } else {
-    T *Wb = g.W + ((i64)batch_id * g.ksplit + ks) * g.m * g.n;
-#pragma unroll
-    for (int a = 0; a < FM; ++a)
-#pragma unroll
-      for (int b = 0; b < FN; ++b)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          const i64 r = m0 + wm * WTM + a * 16 + Mfma<T>::row(lane, q);
-          const i64 c = n0 + wn * WTN + b * 16 + lr;
-          if (r < g.m && c < g.n) Wb[r + c * g.m] = acc[a][b][q];
-        }
+  for (int u = 0; u < FILLS; ++u) {
+    const int f = 8 * u + wave;
+    if (u < A_FILLS / 8) {
+      if (KA) {
+        const int row = 8 * f + (lane >> 3), c = (lane & 7) ^ swz(row);
+        const i64 gi = (m0 + row < g.m) ? m0 + row : g.m - 1;
+        src[u] = Ab + gi * g.lda + kbeg + 2 * c;
+        loff[u] = f * 1024;
+      } else {
+        const i64 i = (m0 + 2 * lane <= g.m - 2) ? m0 + 2 * lane : g.m - 2;
+        src[u] = Ab + i + (kbeg + f) * g.lda;
+        loff[u] = f * PITCH_A;
+      }
+    } else {
+      const int fb = f - A_FILLS;
+      if (KB) {
+        const int row = 8 * fb + (lane >> 3), c = (lane & 7) ^ swz(row);
+        const i64 gj = (n0 + row < g.n) ? n0 + row : g.n - 1;
+        src[u] = Bb + gj * g.ldb + kbeg + 2 * c;
+        loff[u] = L::A_BYTES + fb * 1024;
+      } else {
+        const int q = fb >> 1, h = fb & 1;
+        const i64 j = (n0 + 128 * h + 2 * lane <= g.n - 2) ? n0 + 128 * h + 2 * lane : g.n - 2;
+        src[u] = Bb + j + (kbeg + q) * g.ldb;
+        loff[u] = L::A_BYTES + q * PITCH_B + h * 1024;
+      }
+    }
   }
+  const i64 dA = KA ? BK : BK * g.lda, dB = KB ? BK : BK * g.ldb;  // per K step
+  auto fill = [&](i64 kt, unsigned char *st) {
+    const unsigned base = lds_addr(st);
+#pragma unroll
+    for (int u = 0; u < FILLS; ++u)
+      lds_fill16(src[u] + kt * (u < A_FILLS / 8 ? dA : dB),
+                 __builtin_amdgcn_readfirstlane(base + loff[u]));
+  };
+  // LDS byte offset of op(A)[row][k] / op(B)[k][col] in a stage
+  auto a_off = [](int row, int k) {
+    return KA ? row * 128 + (((k >> 1) ^ swz(row)) << 4) + (k & 1) * 8 : k * PITCH_A + row * 8;
+  };
+  auto b_off = [](int col, int k) {
+    return L::A_BYTES +
+           (KB ? col * 128 + (((k >> 1) ^ swz(col)) << 4) + (k & 1) * 8 : k * PITCH_B + col * 8);
+  };
+  // the partial last K step: registers with bounds checks, zeros past kend
+  auto fill_checked = [&](i64 k0, unsigned char *st) {
+    for (int e = t; e < BM * BK; e += NT) {
+      const int row = KA ? e >> 4 : e % BM, k = KA ? e & 15 : e / BM;
+      const i64 gi = m0 + row, gk = k0 + k;
+      double v = 0.0;
+      if (gi < g.m && gk < kend) v = KA ? Ab[gi * g.lda + gk] : Ab[gi + gk * g.lda];
+      *reinterpret_cast<double *>(st + a_off(row, k)) = v;
+    }
+    for (int e = t; e < BN * BK; e += NT) {
+      const int col = KB ? e >> 4 : e % BN, k = KB ? e & 15 : e / BN;
+      const i64 gj = n0 + col, gk = k0 + k;
+      double v = 0.0;
+      if (gj < g.n && gk < kend) v = KB ? Bb[gk + gj * g.ldb] : Bb[gj + gk * g.ldb];
+      *reinterpret_cast<double *>(st + b_off(col, k)) = v;
+    }
+  };
+
+  d4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = d4{0, 0, 0, 0};
+
+  auto compute = [&](const unsigned char *st) {
+#pragma unroll
+    for (int k8 = 0; k8 < BK; k8 += 8) {
+      const int k = k8 + 2 * lk;  // this lane's K pair
+      double fa[4][2], fb[4][2];
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+        const int r = wm * 64 + 16 * a + lr;
+        if (KA) {
+          const d2 v = *reinterpret_cast<const d2 *>(st + a_off(r, k));
+          fa[a][0] = v.x;
+          fa[a][1] = v.y;
+        } else {
+          fa[a][0] = *reinterpret_cast<const double *>(st + a_off(r, k));
+          fa[a][1] = *reinterpret_cast<const double *>(st + a_off(r, k + 1));
+        }
+      }
+#pragma unroll
+      for (int b = 0; b < 4; ++b) {
+        const int c = wn * 64 + 16 * b + lr;
+        if (KB) {
+          const d2 v = *reinterpret_cast<const d2 *>(st + b_off(c, k));
+          fb[b][0] = v.x;
+          fb[b][1] = v.y;
+        } else {
+          fb[b][0] = *reinterpret_cast<const double *>(st + b_off(c, k));
+          fb[b][1] = *reinterpret_cast<const double *>(st + b_off(c, k + 1));
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+          for (int b = 0; b < 4; ++b)
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a][h], fb[b][h], acc[a][b], 0,
+                                                              0, 0);
+    }
+  };
+
+  const i64 nkf = (kend - kbeg) / BK;  // full K steps
+  const bool tail = ((kend - kbeg) % BK) != 0;
+  // prologue: steps 0 .. S - 2 into stages 0 .. S - 2
+  if (nkf > 0) fill(0, st0);
+  if (nkf > 1) fill(1, st1);
+  // step kt uses stage kt % S and refills stage (kt - 1) % S with step kt + S - 1
+  auto step = [&](i64 kt, auto k) {
+    constexpr int kk = decltype(k)::value;
+    if (kt + S - 2 < nkf) __builtin_amdgcn_s_waitcnt(waitcnt_vm(FILLS * (S - 2)));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
+    // every wave's fills of step kt are in, and every wave is done with step kt - 1
+    bare_barrier();
+    if (kt + S - 1 < nkf)
+      fill(kt + S - 1, stage(std::integral_constant<int, (kk + S - 1) % S>{}));
+    compute(stage(k));
+  };
+  for (i64 kb = 0; kb < nkf; kb += S) {
+    step(kb, std::integral_constant<int, 0>{});
+    if (kb + 1 < nkf) step(kb + 1, std::integral_constant<int, 1>{});
+    if (kb + 2 < nkf) step(kb + 2, std::integral_constant<int, 2>{});
+  }
+  if (tail) {
+    __syncthreads();  // every wave is done with the stages
+    fill_checked(kbeg + nkf * BK, st0);
+    __syncthreads();
+    compute(st0);
+  }
+  store_tile<double, 4, 4>(g, acc, m0 + wm * 64, n0 + wn * 64, lane, batch_id, ks);
+}
+
+// Whether gemm_dl_kernel<KA, KB> can take this product (every fill is an aligned 16-byte
+// read inside the operands).
+inline bool dl_eligible(const GemmArgs<double> &g, bool KA, bool KB, i64 batch) {
+  auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  if (!aligned(g.A) || !aligned(g.B)) return false;
+  if ((g.lda & 1) || (g.ldb & 1)) return false;
+  if (batch > 1 && ((g.sA & 1) || (g.sB & 1))) return false;
+  if (!KA && (g.m & 1)) return false;
+  if (!KB && (g.n & 1)) return false;
+  return true;
 }
 
 // C = alpha * sum_s W[s] + beta * C  (fixed summation order -> deterministic).
@@ -333,8 +576,17 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
     g.W = static_cast<T *>(splitk_workspace(sizeof(T) * (size_t)(batch * ksplit * g.m * g.n)));
   ISVD_CHECK(batch * ksplit <= 65535, "grid.y too large (batch %lld x ksplit %d)", batch, ksplit);
   dim3 grid((unsigned)tiles, (unsigned)(batch * ksplit));
-  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, BM, BN, BK, WGM, WGN>), grid, dim3(64 * WGM * WGN),
-                     0, ctx().stream, g);
+  bool done = false;
+  // (op(A) = A^T with op(B) = B^T, unused by the stages, stays on the register-staged kernel)
+  if constexpr (std::is_same_v<T, double> && BM == dl::BM && BN == dl::BN && !(TA && TB)) {
+    if (dl_eligible(g, TA, !TB, batch)) {
+      hipLaunchKernelGGL((gemm_dl_kernel<TA, !TB>), grid, dim3(dl::NT), 0, ctx().stream, g);
+      done = true;
+    }
+  }
+  if (!done)
+    hipLaunchKernelGGL((gemm_kernel<T, TA, TB, BM, BN, BK, WGM, WGN>), grid,
+                       dim3(64 * WGM * WGN), 0, ctx().stream, g);
   LAUNCH_CHECK();
   if (ksplit > 1) {
     const i64 mn = g.m * g.n;
