@@ -339,7 +339,7 @@ def main():
     achieved = flop / t_sketch
     roofline = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK_FP64_MFMA / 1e12,
                 "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_MFMA, "traffic": None,
-                "kernel": "isvd::gemm_kernel<double,false,false,128,256,16,2,4> "
+                "kernel": "isvd::gemm_dl_kernel<false,true> "
                           "(sketch Y^T = Omega^T A^T, A row-major)",
                 "flop_per_launch": flop, "avg_launch_seconds": t_sketch}
     prof = os.path.join(ROOT, "profiles", "sketch_gemm_fp64_131072x16384_traffic.json")

@@ -59,6 +59,14 @@ void ensure_init() {
   // A blocking stream: work the caller queued on the legacy default stream (e.g. the kernels
   // that produced device-resident inputs) completes before the library's kernels start.
   HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamDefault));
+  // Stream-ordered buffers come from the device's default pool.  Keep freed memory cached in
+  // the pool across synchronizations (the integration loops synchronize once per iteration):
+  // with the default threshold of 0 every synchronization returns it to the driver and the
+  // next allocation maps it again (milliseconds).  isvd_Finalize_gpu trims the pool.
+  hipMemPool_t pool = nullptr;
+  HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, 0));
+  uint64_t keep = UINT64_MAX;
+  HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
   g_init = true;
 }
 
@@ -820,6 +828,9 @@ void isvd_Finalize_gpu(void) {
       g_ws_bytes = 0;
       HIP_CHECK(hipStreamDestroy(g_ctx.stream));
       g_ctx.stream = nullptr;
+      hipMemPool_t pool = nullptr;
+      HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, 0));
+      HIP_CHECK(hipMemPoolTrimTo(pool, 0));
       g_init = false;
     }
   });
