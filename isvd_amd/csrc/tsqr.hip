@@ -17,13 +17,16 @@
 
 #include "dpp.hpp"
 
+#include <utility>
+
 namespace isvd {
 
 namespace {
 
 // Panel layout: 8 waves; wave w owns columns [w*cpw, (w+1)*cpw) of the panel (cpw = LC / 8,
-// LC = l rounded up to a power of two >= 8).  Inside a wave, lane = 16 * cs + rg: the lane
-// holds rows rg + 16 i (i < 16) of the columns w*cpw + cs + 4 t (t < 4).  Column norms and
+// LC = l rounded up to a power of two >= 32, so cpw is 4, 8 or 16).  Inside a wave,
+// lane = 16 * cs + rg: the lane holds rows rg + 16 i (i < 16) of the columns
+// w*cpw + 4 cs + t (t < 4), so column j sits in register slot t = j % 4 of its lanes.  Column norms and
 // the products v^T A[:, c] are reductions over the 16 lanes that share cs (four shuffle
 // levels, issued level by level for all of the lane's columns at once); the Householder
 // vector of each step is broadcast through a double-buffered LDS vector, one barrier per step.
@@ -107,7 +110,7 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
 
   // column of slot t for this lane, or -1
   auto colof = [&](int t) {
-    const int lc = cs + 4 * t;
+    const int lc = 4 * cs + t;
     const int c = wave * cpw + lc;
     return (lc < cpw && c < l) ? c : -1;
   };
@@ -125,25 +128,18 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
   T *tau_out = reinterpret_cast<T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
   __syncthreads();
 
-  for (int j = 0; j < kmax; ++j) {
+  // one Householder step; column j is in register slot TJ = j % 4 of the lanes that hold it
+  auto hh_step = [&](int j, auto tj_c) {
+    constexpr int TJ = decltype(tj_c)::value;
     const int buf = j & 1;
     const int lj = j - wave * cpw;  // column j relative to this wave
-    if (lj >= 0 && lj < cpw && cs == (lj & 3)) {
-      const int tj = lj >> 2;
-      T col[kRPL];
-#pragma unroll
-      for (int i = 0; i < kRPL; ++i) col[i] = T(0);
-#pragma unroll
-      for (int t = 0; t < kCPL; ++t)
-        if (t == tj)
-#pragma unroll
-          for (int i = 0; i < kRPL; ++i) col[i] = x[t][i];
+    if (lj >= 0 && lj < cpw && cs == (lj >> 2)) {
       T part = 0, av = 0;
 #pragma unroll
       for (int i = 0; i < kRPL; ++i) {
         const int r = rg + 16 * i;
-        if (r > j) part += col[i] * col[i];  // rows >= h hold zeros
-        if (r == j) av = col[i];
+        if (r > j) part += x[TJ][i] * x[TJ][i];  // rows >= h hold zeros
+        if (r == j) av = x[TJ][i];
       }
       const T sigma2 = group16_sum1(part);
       const T alpha = group16_sum1(av);  // one lane contributes
@@ -159,15 +155,10 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
 #pragma unroll
       for (int i = 0; i < kRPL; ++i) {
         const int r = rg + 16 * i;
-        if (r > j) col[i] *= scale;
-        else if (r == j) col[i] = beta;
-        vsh[buf][r] = (r > j && r < h) ? col[i] : (r == j ? T(1) : T(0));
+        if (r > j) x[TJ][i] *= scale;
+        else if (r == j) x[TJ][i] = beta;
+        vsh[buf][r] = (r > j && r < h) ? x[TJ][i] : (r == j ? T(1) : T(0));
       }
-#pragma unroll
-      for (int t = 0; t < kCPL; ++t)
-        if (t == tj)
-#pragma unroll
-          for (int i = 0; i < kRPL; ++i) x[t][i] = col[i];
       if (rg == 0) {
         tau_out[j] = tau;
         tau_sh[buf] = tau;
@@ -196,6 +187,13 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
         }
       }
     }
+  };
+  // cpw is a multiple of 4, so j % 4 is the slot of column j in whichever wave holds it
+  for (int j0 = 0; j0 < kmax; j0 += 4) {
+    hh_step(j0, std::integral_constant<int, 0>{});
+    if (j0 + 1 < kmax) hh_step(j0 + 1, std::integral_constant<int, 1>{});
+    if (j0 + 2 < kmax) hh_step(j0 + 2, std::integral_constant<int, 2>{});
+    if (j0 + 3 < kmax) hh_step(j0 + 3, std::integral_constant<int, 3>{});
   }
   // V (below the diagonal) and R (on and above) back in place; R to the next level
   T *R = reinterpret_cast<T *>(a.Rout) + ((i64)b * a.nnode + j_node) * l * l;
@@ -230,7 +228,7 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
   const T *C = reinterpret_cast<const T *>(a.Cin) + ((i64)b * a.nnode + j_node) * l * l;
   const T *tau_in = reinterpret_cast<const T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
   auto colof = [&](int t) {
-    const int lc = cs + 4 * t;
+    const int lc = 4 * cs + t;
     const int c = wave * cpw + lc;
     return (lc < cpw && c < l) ? c : -1;
   };
@@ -334,7 +332,7 @@ void tsqr_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr
   ISVD_CHECK(l >= 1 && l <= 128, "TSQR supports 1 <= l <= 128 (got %lld)", l);
   ISVD_CHECK(m >= l, "TSQR needs m >= l (m = %lld, l = %lld)", m, l);
   ISVD_CHECK(batch >= 1 && batch <= 65535, "bad batch %lld", batch);
-  int LC = kWaves;
+  int LC = 4 * kWaves;  // >= 4 columns per wave: column j is in slot j % 4 of its lanes
   while (LC < l) LC *= 2;
   const i64 hmax = kHMax;  // rows one workgroup holds
   ISVD_CHECK(hmax >= 2 * l && LC / kWaves <= kMaxCPW, "internal: panel capacity");
