@@ -210,11 +210,17 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
   }
 }
 
-// Explicit Q: X := H_0 H_1 ... H_{k-1} [C; 0], reflectors applied last-to-first.  The vector of
-// step j - 1 is loaded from the panel by one wave while step j computes.
+// Explicit Q: X := H_0 H_1 ... H_{k-1} [C; 0], reflectors applied last-to-first.  Every
+// reflector is known before the pass starts, so the waves need no step-by-step agreement: the
+// reflectors move through LDS in blocks of kVB (vectors expanded to full panel height, zeros
+// above the unit diagonal and below the panel), double-buffered, with the next block's global
+// loads in flight in registers while every wave applies the current block to its own columns.
+// One workgroup barrier per block.
+constexpr int kVB = 16;                       // reflectors per LDS block
+constexpr int kVLoad = kVB * kHMax / kNT;     // entries each thread loads per block
 template <typename T>
 __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
-  __shared__ T vsh[2][kHMax];
+  __shared__ T vsh[2][kVB][kHMax + 1];
   const int j_node = blockIdx.x, b = blockIdx.y;
   const int l = a.l, cpw = a.LC / kWaves;
   // the wave index is uniform: keep it, and what depends on it, in scalar registers
@@ -242,29 +248,42 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
       x[t][i] = (c >= 0 && r < kmax) ? C[r * l + c] : T(0);
     }
   }
-  // v_j[r] = 0 (r < j), 1 (r == j), V[r, j] (j < r < h), 0 (r >= h); 64 lanes x 4 entries
-  auto vload = [&](int j, int r) -> T {
-    return (r < j || r >= h) ? T(0) : (r == j ? T(1) : base[(i64)r * ld + j]);
-  };
-  if (kmax > 0 && wave == 0) {
+  // entry q of this thread's share of a block: row r = e / kVB, reflector jj = e % kVB
+  // (e = tid + kNT q), so that consecutive threads read consecutive columns of one panel row;
+  // v_j[r] = 0 (r < j), 1 (r == j), V[r, j] (j < r < h), 0 (r >= h).  LDS rows are padded by
+  // one entry so that the transposing stores fall in distinct banks.
+  T vreg[kVLoad];
+  auto vload = [&](int j0) {
 #pragma unroll
-    for (int q = 0; q < kHMax / 64; ++q)
-      vsh[(kmax - 1) & 1][lane + 64 * q] = vload(kmax - 1, lane + 64 * q);
-  }
-  __syncthreads();
-  for (int j = kmax - 1; j >= 0; --j) {
-    const int buf = j & 1;
-    const bool loader = (j > 0) && (wave == (j - 1) % kWaves);
-    T vnext[kHMax / 64];
-    if (loader) {
-#pragma unroll
-      for (int q = 0; q < kHMax / 64; ++q) vnext[q] = vload(j - 1, lane + 64 * q);
+    for (int q = 0; q < kVLoad; ++q) {
+      const int e = tid + kNT * q, r = e / kVB, jj = e % kVB, j = j0 + jj;
+      vreg[q] = (j >= kmax || r < j || r >= h) ? T(0)
+                                               : (r == j ? T(1) : base[(i64)r * ld + j]);
     }
-    const T tau = tau_in[j];
-    if (tau != T(0)) {
+  };
+  auto vstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < kVLoad; ++q) {
+      const int e = tid + kNT * q;
+      vsh[buf][e % kVB][e / kVB] = vreg[q];
+    }
+  };
+  const int nblk = (kmax + kVB - 1) / kVB;
+  if (nblk > 0) {
+    vload((nblk - 1) * kVB);
+    vstore((nblk - 1) & 1);
+  }
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int buf = blk & 1, j0 = blk * kVB;
+    __syncthreads();  // block blk is in LDS; every wave is done with block blk + 1's buffer
+    if (blk > 0) vload(j0 - kVB);
+    const int jl = (j0 + kVB < kmax ? j0 + kVB : kmax) - 1;
+    for (int j = jl; j >= j0; --j) {
+      const T tau = tau_in[j];
+      if (tau == T(0)) continue;
       T v[kRPL];
 #pragma unroll
-      for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][rg + 16 * i];
+      for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][j - j0][rg + 16 * i];
       T part[kCPL];
 #pragma unroll
       for (int t = 0; t < kCPL; ++t) {
@@ -280,11 +299,7 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
         for (int i = 0; i < kRPL; ++i) x[t][i] -= v[i] * w;
       }
     }
-    if (loader) {
-#pragma unroll
-      for (int q = 0; q < kHMax / 64; ++q) vsh[buf ^ 1][lane + 64 * q] = vnext[q];
-    }
-    __syncthreads();
+    if (blk > 0) vstore(buf ^ 1);
   }
   T *out;
   i64 ldo;
@@ -294,6 +309,8 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
     out = reinterpret_cast<T *>(a.Cout) + ((i64)b * a.nchild + 2 * j_node) * l * l;
     ldo = l;
   }
+  // every wave has read its V entries from the panel before any wave overwrites it with Q
+  __syncthreads();
 #pragma unroll
   for (int t = 0; t < kCPL; ++t) {
     const int c = colof(t);
