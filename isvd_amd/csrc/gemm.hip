@@ -60,6 +60,7 @@ struct GemmArgs {
   T *W;          // split-K workspace: [batch][ksplit][m*n] dense col-major partials
   int group_m;   // raster group height (tiles) for L2 reuse
   int upper;     // 1: only tiles touching the upper triangle (symmetric result, mirrored after)
+  int ct;        // 1: the result is stored transposed, C[c + r ldc] (n x m)
 };
 
 // Tile (tm, tn) of this workgroup; false when it has none.  Workgroups are dispatched
@@ -104,8 +105,8 @@ __device__ inline bool tile_coords(const GemmArgs<T> &g, int BM, int BN, i64 &tm
 }
 
 // Epilogue of one wave: fragment (a, b) holds rows r0 + 16 a + Mfma::row(lane, q) and column
-// c0 + 16 b + lane % 16.  Without split-K, C := alpha acc + beta C; with it, the raw partial
-// goes to the workspace slice of chunk ks (dense m x n).
+// c0 + 16 b + lane % 16.  Without split-K, C := alpha acc + beta C (C^T when g.ct); with it,
+// the raw partial goes to the workspace slice of chunk ks (dense m x n).
 template <typename T, int FM, int FN>
 __device__ inline void store_tile(const GemmArgs<T> &g,
                                   const typename Mfma<T>::acc_t (&acc)[FM][FN], i64 r0, i64 c0,
@@ -122,9 +123,10 @@ __device__ inline void store_tile(const GemmArgs<T> &g,
           const i64 r = r0 + a * 16 + Mfma<T>::row(lane, q);
           const i64 c = c0 + b * 16 + lr;
           if (r < g.m && c < g.n) {
+            const i64 at = g.ct ? c + r * g.ldc : r + c * g.ldc;
             T v = g.alpha * acc[a][b][q];
-            if (g.beta != T(0)) v += g.beta * Cb[r + c * g.ldc];
-            Cb[r + c * g.ldc] = v;
+            if (g.beta != T(0)) v += g.beta * Cb[at];
+            Cb[at] = v;
           }
         }
   } else {
@@ -477,7 +479,8 @@ inline bool dl_eligible(const GemmArgs<double> &g, bool KA, bool KB, i64 batch) 
 // C = alpha * sum_s W[s] + beta * C  (fixed summation order -> deterministic).
 template <typename T>
 __global__ void splitk_reduce_kernel(i64 m, i64 n, int ksplit, const T *__restrict__ W,
-                                     T alpha, T beta, T *__restrict__ C, i64 ldc, i64 sC) {
+                                     T alpha, T beta, T *__restrict__ C, i64 ldc, i64 sC,
+                                     int ct) {
   const int b = blockIdx.y;
   const i64 mn = m * n;
   const T *Wb = W + (i64)b * ksplit * mn;
@@ -487,9 +490,10 @@ __global__ void splitk_reduce_kernel(i64 m, i64 n, int ksplit, const T *__restri
     T s = T(0);
     for (int q = 0; q < ksplit; ++q) s += Wb[q * mn + e];
     const i64 r = e % m, c = e / m;
+    const i64 at = ct ? c + r * ldc : r + c * ldc;
     T v = alpha * s;
-    if (beta != T(0)) v += beta * Cb[r + c * ldc];
-    Cb[r + c * ldc] = v;
+    if (beta != T(0)) v += beta * Cb[at];
+    Cb[at] = v;
   }
 }
 
@@ -594,7 +598,7 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(splitk_reduce_kernel<T>, dim3(blocks, (unsigned)batch), dim3(256), 0,
                        ctx().stream, g.m, g.n, ksplit, (const T *)g.W, g.alpha, g.beta, g.C,
-                       g.ldc, g.sC);
+                       g.ldc, g.sC, g.ct);
     LAUNCH_CHECK();
   }
 }
@@ -620,16 +624,16 @@ void launch(GemmArgs<T> g, i64 batch) {
 template <typename T>
 static void gemm_impl(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A,
                       i64 lda, const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch, i64 sA,
-                      i64 sB, i64 sC, int upper) {
+                      i64 sB, i64 sC, int upper, int ct = 0) {
   if (m <= 0 || n <= 0 || batch <= 0) return;
   const bool ta = (transa == 'T' || transa == 't' || transa == 'C' || transa == 'c');
   const bool tb = (transb == 'T' || transb == 't' || transb == 'C' || transb == 'c');
-  ISVD_CHECK(ldc >= m, "ldc (%lld) < m (%lld)", ldc, m);
+  ISVD_CHECK(ldc >= (ct ? n : m), "ldc (%lld) too small", ldc);
   if (k <= 0 || alpha == T(0)) {
     int blocks = (int)((m * n + 255) / 256);
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(scale_kernel<T>, dim3(blocks, (unsigned)batch), dim3(256), 0,
-                       ctx().stream, m, n, beta, C, ldc, sC);
+                       ctx().stream, ct ? n : m, ct ? m : n, beta, C, ldc, sC);
     LAUNCH_CHECK();
     return;
   }
@@ -638,7 +642,7 @@ static void gemm_impl(char transa, char transb, i64 m, i64 n, i64 k, T alpha, co
   GemmArgs<T> g{};
   g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
   g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
-  g.sA = sA; g.sB = sB; g.sC = sC; g.W = nullptr; g.upper = upper;
+  g.sA = sA; g.sB = sB; g.sC = sC; g.W = nullptr; g.upper = upper; g.ct = ct;
   if (!ta && !tb) launch<T, false, false>(g, batch);
   else if (!ta && tb) launch<T, false, true>(g, batch);
   else if (ta && !tb) launch<T, true, false>(g, batch);
@@ -650,6 +654,12 @@ void gemm(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i6
           const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch, i64 sA, i64 sB, i64 sC) {
   gemm_impl<T>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, batch, sA, sB, sC,
                0);
+}
+
+template <typename T>
+void gemm_ct(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda,
+             const T *B, i64 ldb, T beta, T *C, i64 ldc) {
+  gemm_impl<T>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 1, 0, 0, 0, 0, 1);
 }
 
 template <typename T>
@@ -668,6 +678,10 @@ void syrk_full(char trans, i64 n, i64 k, T alpha, const T *A, i64 lda, T *C, i64
 template void syrk_full<double>(char, i64, i64, double, const double *, i64, double *, i64);
 template void syrk_full<float>(char, i64, i64, float, const float *, i64, float *, i64);
 
+template void gemm_ct<double>(char, char, i64, i64, i64, double, const double *, i64,
+                              const double *, i64, double, double *, i64);
+template void gemm_ct<float>(char, char, i64, i64, i64, float, const float *, i64,
+                             const float *, i64, float, float *, i64);
 template void gemm<double>(char, char, i64, i64, i64, double, const double *, i64,
                            const double *, i64, double, double *, i64, i64, i64, i64, i64);
 template void gemm<float>(char, char, i64, i64, i64, float, const float *, i64, const float *,

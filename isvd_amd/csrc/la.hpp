@@ -17,6 +17,13 @@ void gemm(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i6
           const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch = 1, i64 sA = 0, i64 sB = 0,
           i64 sC = 0);
 
+// C^T := alpha * (op(A) * op(B))^T + beta * C^T: the same product stored transposed, i.e. C
+// is n x m with leading dimension ldc >= n.  (M B for a symmetric M is (B^T M)^T, which takes
+// the wide-tile kernel when B has few columns.)
+template <typename T>
+void gemm_ct(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda,
+             const T *B, i64 ldb, T beta, T *C, i64 ldc);
+
 // C := alpha op(A) op(A)^T (n x n, full storage): op(A) = A (trans 'N', A n x k) or A^T
 // (trans 'T', A k x n).  Computes the upper-triangle tiles and mirrors them.
 template <typename T>

@@ -307,8 +307,8 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
   auto prep = [&](int q) {
     T *X = xb[q], *B = X + Nl * l, *Bg = X + 2 * Nl * l;
     (void)X;
-    // Bg := M B / N  (:174-178)
-    gemm<T>('N', 'N', Nl, l, Nl, ninv, M, Nl, B, Nl, T(0), Bg, Nl);
+    // Bg := M B / N  (:174-178), as (B^T M)^T: M is exactly symmetric
+    gemm_ct<T>('T', 'N', l, Nl, Nl, ninv, B, Nl, M, Nl, T(0), Bg, Nl);
     // D := B^T B / N (:181), Dg := B^T Bg / N (:184-186)
     gemm<T>('T', 'N', l, 2 * l, Nl, ninv, B, Nl, B, Nl, T(0), D, l);
     // Z := Dg - D^T D (:187-190)
@@ -405,7 +405,7 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
   omatcopy<T>('N', Nl, l, T(1), M, Nl, bc.get(), Nl);
   gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bc.get(), Nl, T(0), dc.get(), l);
   omatcopy<T>('N', Nl, l, ninv, bc.get(), Nl, gq.get(), Nl);
-  gemm<T>('N', 'N', Nl, l, Nl, T(1), M, Nl, gq.get(), Nl, T(0), bgc.get(), Nl);
+  gemm_ct<T>('T', 'N', l, Nl, Nl, T(1), gq.get(), Nl, M, Nl, T(0), bgc.get(), Nl);
   gemm<T>('T', 'N', l, l, Nl, ninv, bc.get(), Nl, bgc.get(), Nl, T(0), dgc.get(), l);
   omatcopy<T>('N', Nl, l, T(1), gq.get(), Nl, xx.get(), Nl);
   gemm<T>('N', 'N', Nl, l, l, T(-1), xq.get(), Nl, dc.get(), l, T(1), xx.get(), Nl);
@@ -462,7 +462,7 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
     // D+ := B+^T B+ / N (:295); G+ := Qs B+ / N (:298); Bg+ := Qs^T G+ (:301-303)
     gemm<T>('T', 'N', l, l, Nl, ninv, bp.get(), Nl, bp.get(), Nl, T(0), dc.get(), l);
     omatcopy<T>('N', Nl, l, ninv, bp.get(), Nl, gq.get(), Nl);
-    gemm<T>('N', 'N', Nl, l, Nl, T(1), M, Nl, gq.get(), Nl, T(0), bgc.get(), Nl);
+    gemm_ct<T>('T', 'N', l, Nl, Nl, T(1), gq.get(), Nl, M, Nl, T(0), bgc.get(), Nl);
     // Dg+ := B+^T Bg+ / N (:306); mu (:307)
     gemm<T>('T', 'N', l, l, Nl, ninv, bp.get(), Nl, bgc.get(), Nl, T(0), dgc.get(), l);
     hipLaunchKernelGGL(wy_mu_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
@@ -485,8 +485,8 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
                        Nl * l, (const T *)xx.get(), (const T *)xxp.get(), d2.get());
     LAUNCH_CHECK();
     // t (:334-342) in sketch coordinates: <D_a, D_b> = <d_a, M d_b>
-    gemm<T>('N', 'N', Nl, l, Nl, T(1), M, Nl, d1.get(), Nl, T(0), md1.get(), Nl);
-    gemm<T>('N', 'N', Nl, l, Nl, T(1), M, Nl, d2.get(), Nl, T(0), md2.get(), Nl);
+    gemm_ct<T>('T', 'N', l, Nl, Nl, T(1), d1.get(), Nl, M, Nl, T(0), md1.get(), Nl);
+    gemm_ct<T>('T', 'N', l, Nl, Nl, T(1), d2.get(), Nl, M, Nl, T(0), md2.get(), Nl);
     if (it % 2) {
       dot<T>(Nl * l, d1.get(), md1.get(), scal.get());
       dot<T>(Nl * l, d1.get(), md2.get(), scal.get() + 1);
