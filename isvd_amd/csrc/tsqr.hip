@@ -166,7 +166,8 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
     }
     __syncthreads();
     const T tau = tau_sh[buf];
-    if (tau != T(0)) {
+    // a wave whose columns all lie left of j + 1 has nothing to update
+    if (tau != T(0) && wave * cpw + cpw - 1 > j) {
       T v[kRPL];
 #pragma unroll
       for (int i = 0; i < kRPL; ++i) v[i] = vsh[buf][rg + 16 * i];
