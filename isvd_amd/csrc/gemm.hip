@@ -323,33 +323,37 @@ __global__ __launch_bounds__(dl::NT) void gemm_dl_kernel(GemmArgs<double> g) {
 
   // fill slot u of this wave is fill instruction f = 8 u + wave of a stage: op(A) for
   // f < A_FILLS, op(B) after
-  const double *src[FILLS];
-  unsigned loff[FILLS];
+  // wave-uniform operand bases at this chunk's first K step; per lane, a 32-bit element
+  // offset for each fill slot (the host keeps the leading dimensions below 2^23)
+  const double *abase = Ab + (KA ? m0 * g.lda + kbeg : m0 + kbeg * g.lda);
+  const double *bbase = Bb + (KB ? n0 * g.ldb + kbeg : n0 + kbeg * g.ldb);
+  unsigned off[FILLS], loff[FILLS];
 #pragma unroll
   for (int u = 0; u < FILLS; ++u) {
     const int f = 8 * u + wave;
     if (u < A_FILLS / 8) {
       if (KA) {
         const int row = 8 * f + (lane >> 3), c = (lane & 7) ^ swz(row);
-        const i64 gi = (m0 + row < g.m) ? m0 + row : g.m - 1;
-        src[u] = Ab + gi * g.lda + kbeg + 2 * c;
+        const i64 gi = (m0 + row < g.m) ? row : g.m - 1 - m0;
+        off[u] = (unsigned)(gi * g.lda + 2 * c);
         loff[u] = f * 1024;
       } else {
-        const i64 i = (m0 + 2 * lane <= g.m - 2) ? m0 + 2 * lane : g.m - 2;
-        src[u] = Ab + i + (kbeg + f) * g.lda;
+        const i64 i = (m0 + 2 * lane <= g.m - 2) ? 2 * lane : g.m - 2 - m0;
+        off[u] = (unsigned)(i + f * g.lda);
         loff[u] = f * PITCH_A;
       }
     } else {
       const int fb = f - A_FILLS;
       if (KB) {
         const int row = 8 * fb + (lane >> 3), c = (lane & 7) ^ swz(row);
-        const i64 gj = (n0 + row < g.n) ? n0 + row : g.n - 1;
-        src[u] = Bb + gj * g.ldb + kbeg + 2 * c;
+        const i64 gj = (n0 + row < g.n) ? row : g.n - 1 - n0;
+        off[u] = (unsigned)(gj * g.ldb + 2 * c);
         loff[u] = L::A_BYTES + fb * 1024;
       } else {
         const int q = fb >> 1, h = fb & 1;
-        const i64 j = (n0 + 128 * h + 2 * lane <= g.n - 2) ? n0 + 128 * h + 2 * lane : g.n - 2;
-        src[u] = Bb + j + (kbeg + q) * g.ldb;
+        const i64 j =
+            (n0 + 128 * h + 2 * lane <= g.n - 2) ? 128 * h + 2 * lane : g.n - 2 - n0;
+        off[u] = (unsigned)(j + q * g.ldb);
         loff[u] = L::A_BYTES + q * PITCH_B + h * 1024;
       }
     }
@@ -357,9 +361,10 @@ __global__ __launch_bounds__(dl::NT) void gemm_dl_kernel(GemmArgs<double> g) {
   const i64 dA = KA ? BK : BK * g.lda, dB = KB ? BK : BK * g.ldb;  // per K step
   auto fill = [&](i64 kt, unsigned char *st) {
     const unsigned base = lds_addr(st);
+    const double *ak = abase + kt * dA, *bk = bbase + kt * dB;
 #pragma unroll
     for (int u = 0; u < FILLS; ++u)
-      lds_fill16(src[u] + kt * (u < A_FILLS / 8 ? dA : dB),
+      lds_fill16((u < A_FILLS / 8 ? ak : bk) + off[u],
                  __builtin_amdgcn_readfirstlane(base + loff[u]));
   };
   // LDS byte offset of op(A)[row][k] / op(B)[k][col] in a stage
@@ -394,44 +399,43 @@ __global__ __launch_bounds__(dl::NT) void gemm_dl_kernel(GemmArgs<double> g) {
 #pragma unroll
     for (int b = 0; b < 4; ++b) acc[a][b] = d4{0, 0, 0, 0};
 
-  auto compute = [&](const unsigned char *st) {
+  // fragments of one half step (8 K values): lane l / 16 = s holds K index k8 + 2 s + h for
+  // instruction h of each pair
+  typedef double Frag[4][2];
+  auto load = [&](const unsigned char *st, int k8, Frag &fa, Frag &fb) {
+    const int k = k8 + 2 * lk;  // this lane's K pair
 #pragma unroll
-    for (int k8 = 0; k8 < BK; k8 += 8) {
-      const int k = k8 + 2 * lk;  // this lane's K pair
-      double fa[4][2], fb[4][2];
-#pragma unroll
-      for (int a = 0; a < 4; ++a) {
-        const int r = wm * 64 + 16 * a + lr;
-        if (KA) {
-          const d2 v = *reinterpret_cast<const d2 *>(st + a_off(r, k));
-          fa[a][0] = v.x;
-          fa[a][1] = v.y;
-        } else {
-          fa[a][0] = *reinterpret_cast<const double *>(st + a_off(r, k));
-          fa[a][1] = *reinterpret_cast<const double *>(st + a_off(r, k + 1));
-        }
+    for (int a = 0; a < 4; ++a) {
+      const int r = wm * 64 + 16 * a + lr;
+      if (KA) {
+        const d2 v = *reinterpret_cast<const d2 *>(st + a_off(r, k));
+        fa[a][0] = v.x;
+        fa[a][1] = v.y;
+      } else {
+        fa[a][0] = *reinterpret_cast<const double *>(st + a_off(r, k));
+        fa[a][1] = *reinterpret_cast<const double *>(st + a_off(r, k + 1));
       }
-#pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const int c = wn * 64 + 16 * b + lr;
-        if (KB) {
-          const d2 v = *reinterpret_cast<const d2 *>(st + b_off(c, k));
-          fb[b][0] = v.x;
-          fb[b][1] = v.y;
-        } else {
-          fb[b][0] = *reinterpret_cast<const double *>(st + b_off(c, k));
-          fb[b][1] = *reinterpret_cast<const double *>(st + b_off(c, k + 1));
-        }
-      }
-#pragma unroll
-      for (int h = 0; h < 2; ++h)
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-          for (int b = 0; b < 4; ++b)
-            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a][h], fb[b][h], acc[a][b], 0,
-                                                              0, 0);
     }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const int c = wn * 64 + 16 * b + lr;
+      if (KB) {
+        const d2 v = *reinterpret_cast<const d2 *>(st + b_off(c, k));
+        fb[b][0] = v.x;
+        fb[b][1] = v.y;
+      } else {
+        fb[b][0] = *reinterpret_cast<const double *>(st + b_off(c, k));
+        fb[b][1] = *reinterpret_cast<const double *>(st + b_off(c, k + 1));
+      }
+    }
+  };
+  // the 16 products of instruction h of a half step
+  auto mma = [&](const Frag &fa, const Frag &fb, int h) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(fa[a][h], fb[b][h], acc[a][b], 0, 0, 0);
   };
 
   const i64 nkf = (kend - kbeg) / BK;  // full K steps
@@ -439,27 +443,47 @@ __global__ __launch_bounds__(dl::NT) void gemm_dl_kernel(GemmArgs<double> g) {
   // prologue: steps 0 .. S - 2 into stages 0 .. S - 2
   if (nkf > 0) fill(0, st0);
   if (nkf > 1) fill(1, st1);
-  // step kt uses stage kt % S and refills stage (kt - 1) % S with step kt + S - 1
+  // Step kt uses stage kt % S and refills stage (kt - 1) % S with step kt + S - 1.  The
+  // pipeline is rotated by a quarter step: the last 16 products of step kt - 1 (operands
+  // already in registers) are issued after barrier kt, while the first fragments of stage kt
+  // are read, so the matrix cores do not wait on LDS after the barrier.
+  Frag fa0, fb0, fa1, fb1;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int h = 0; h < 2; ++h) fa1[a][h] = fb1[a][h] = 0.0;  // nothing deferred before step 0
   auto step = [&](i64 kt, auto k) {
     constexpr int kk = decltype(k)::value;
-    if (kt + S - 2 < nkf) __builtin_amdgcn_s_waitcnt(waitcnt_vm(FILLS * (S - 2)));
-    else __builtin_amdgcn_s_waitcnt(waitcnt_vm(0));
-    // every wave's fills of step kt are in, and every wave is done with step kt - 1
+    // this wave's fills of step kt have landed and its fragment reads of stage kt - 1 are
+    // done (another wave refills that stage after the barrier)
+    if (kt + S - 2 < nkf) __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(FILLS * (S - 2)));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(0));
     bare_barrier();
     if (kt + S - 1 < nkf)
       fill(kt + S - 1, stage(std::integral_constant<int, (kk + S - 1) % S>{}));
-    compute(stage(k));
+    load(stage(k), 0, fa0, fb0);
+    mma(fa1, fb1, 1);
+    mma(fa0, fb0, 0);
+    mma(fa0, fb0, 1);
+    load(stage(k), 8, fa1, fb1);
+    mma(fa1, fb1, 0);
   };
   for (i64 kb = 0; kb < nkf; kb += S) {
     step(kb, std::integral_constant<int, 0>{});
     if (kb + 1 < nkf) step(kb + 1, std::integral_constant<int, 1>{});
     if (kb + 2 < nkf) step(kb + 2, std::integral_constant<int, 2>{});
   }
+  mma(fa1, fb1, 1);  // the deferred products of the last full step (zeros when none)
   if (tail) {
     __syncthreads();  // every wave is done with the stages
     fill_checked(kbeg + nkf * BK, st0);
     __syncthreads();
-    compute(st0);
+    load(st0, 0, fa0, fb0);
+    load(st0, 8, fa1, fb1);
+    mma(fa0, fb0, 0);
+    mma(fa0, fb0, 1);
+    mma(fa1, fb1, 0);
+    mma(fa1, fb1, 1);
   }
   store_tile<double, 4, 4>(g, acc, m0 + wm * 64, n0 + wn * 64, lane, batch_id, ks);
 }
@@ -473,6 +497,8 @@ inline bool dl_eligible(const GemmArgs<double> &g, bool KA, bool KB, i64 batch) 
   if (batch > 1 && ((g.sA & 1) || (g.sB & 1))) return false;
   if (!KA && (g.m & 1)) return false;
   if (!KB && (g.n & 1)) return false;
+  // per-lane fill offsets are 32-bit element counts below 256 leading dimensions
+  if (g.lda >= (1 << 23) || g.ldb >= (1 << 23)) return false;
   return true;
 }
 

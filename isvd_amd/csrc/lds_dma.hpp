@@ -11,6 +11,8 @@ namespace isvd {
 // s_waitcnt immediate (gfx9 encoding): at most v vector-memory operations outstanding,
 // expcnt and lgkmcnt unconstrained
 constexpr int waitcnt_vm(int v) { return (v & 15) | (7 << 4) | (15 << 8) | ((v >> 4) << 14); }
+// the same, and no LDS (or scalar-memory) operation outstanding
+constexpr int waitcnt_vm_lgkm0(int v) { return (v & 15) | (7 << 4) | ((v >> 4) << 14); }
 
 // One 16-byte-per-lane fill of LDS from global memory (global_load_lds_dwordx4): lane l's 16
 // bytes land at lds_base + 16 l.  Issued as inline assembly so that the compiler does not
