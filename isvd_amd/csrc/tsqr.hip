@@ -96,7 +96,9 @@ __device__ __forceinline__ void group16_sum(T (&v)[kCPL]) {
 template <typename T>
 __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
   __shared__ T vsh[2][kHMax];
-  __shared__ T tau_sh[2];
+  // the taus stay in LDS until the end: a global store inside the loop would be waited for at
+  // every step's barrier
+  __shared__ T tau_sh[kHMax];
   const int j_node = blockIdx.x, b = blockIdx.y;
   const int l = a.l, cpw = a.LC / kWaves;
   // the wave index is uniform: keep it, and what depends on it, in scalar registers
@@ -159,13 +161,10 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
         else if (r == j) x[TJ][i] = beta;
         vsh[buf][r] = (r > j && r < h) ? x[TJ][i] : (r == j ? T(1) : T(0));
       }
-      if (rg == 0) {
-        tau_out[j] = tau;
-        tau_sh[buf] = tau;
-      }
+      if (rg == 0) tau_sh[j] = tau;
     }
     __syncthreads();
-    const T tau = tau_sh[buf];
+    const T tau = tau_sh[j];
     // a wave whose columns all lie left of j + 1 has nothing to update
     if (tau != T(0) && wave * cpw + cpw - 1 > j) {
       T v[kRPL];
@@ -196,6 +195,7 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
     if (j0 + 2 < kmax) hh_step(j0 + 2, std::integral_constant<int, 2>{});
     if (j0 + 3 < kmax) hh_step(j0 + 3, std::integral_constant<int, 3>{});
   }
+  for (int e = tid; e < kmax; e += kNT) tau_out[e] = tau_sh[e];
   // V (below the diagonal) and R (on and above) back in place; R to the next level
   T *R = reinterpret_cast<T *>(a.Rout) + ((i64)b * a.nnode + j_node) * l * l;
 #pragma unroll
@@ -207,6 +207,144 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
       const int r = rg + 16 * i;
       if (r < h) base[(i64)r * ld + c] = x[t][i];
       if (r < l) R[r * l + c] = (r <= c && r < kmax) ? x[t][i] : T(0);
+    }
+  }
+}
+
+// Internal tree node: QR of the stacked pair [R1; R2] of upper-triangular l x l factors
+// (LAPACK's triangle-on-triangle case, tpqrt with l = n).  Reflector j is 1 at top row j,
+// nonzero on bottom rows 0..j and zero elsewhere, so step j touches top row j and bottom rows
+// 0..j only.  Top row j is therefore still the input row when step j starts: it is read from
+// memory one step ahead, and the bottom triangle alone is held in registers (8 rows per lane,
+// the panel layout otherwise): half the per-step work of a 2l-row panel.
+// Outputs match hh_factor_kernel's: R of the node (row-major), the taus, and the bottom parts
+// of the reflectors stored over R2 (the top parts below the diagonal are R1's zeros).
+constexpr int kTRPL = 8;  // bottom rows per lane (l <= 128)
+template <typename T>
+__global__ __launch_bounds__(kNT) void tt_factor_kernel(LevelArgs a) {
+  // R1, row j replaced by row j of R at step j; written out at the end (as are the taus), so
+  // that no global memory operation is in flight at a step's barrier
+  __shared__ T r1s[16 * kTRPL][16 * kTRPL + 1];
+  __shared__ T vsh[2][16 * kTRPL];
+  __shared__ T tau_sh[16 * kTRPL];
+  const int j_node = blockIdx.x, b = blockIdx.y;
+  const int l = a.l, cpw = a.LC / kWaves;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int cs = lane >> 4, rg = lane & 15;
+  const int c0 = 2 * j_node;
+  const bool pair = c0 + 1 < a.nchild;  // a lone last child: R2 = 0
+  const T *R1 = reinterpret_cast<const T *>(a.Rin) + ((i64)b * a.nchild + c0) * l * l;
+  T *R2 = reinterpret_cast<T *>(a.Rin) + ((i64)b * a.nchild + c0 + 1) * l * l;
+  T *tau_out = reinterpret_cast<T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
+  T *R = reinterpret_cast<T *>(a.Rout) + ((i64)b * a.nnode + j_node) * l * l;
+  auto colof = [&](int t) {
+    const int lc = 4 * cs + t;
+    const int c = wave * cpw + lc;
+    return (lc < cpw && c < l) ? c : -1;
+  };
+  for (int e = tid; e < l * l; e += kNT) r1s[e / l][e % l] = R1[e];
+  T x[kCPL][kTRPL];
+#pragma unroll
+  for (int t = 0; t < kCPL; ++t) {
+    const int c = colof(t);
+#pragma unroll
+    for (int i = 0; i < kTRPL; ++i) {
+      const int r = rg + 16 * i;
+      x[t][i] = (pair && c >= 0 && r < l) ? R2[r * l + c] : T(0);
+    }
+  }
+  __syncthreads();
+  auto tt_step = [&](int j, auto tj_c) __attribute__((always_inline)) {
+    constexpr int TJ = decltype(tj_c)::value;
+    const int buf = j & 1;
+    // top row j on the lane's columns: still R1's row (earlier steps touch other top rows)
+    T r1[kCPL];
+#pragma unroll
+    for (int t = 0; t < kCPL; ++t) {
+      const int c = colof(t);
+      r1[t] = (c >= 0) ? r1s[j][c] : T(0);
+    }
+    const int lj = j - wave * cpw;
+    if (lj >= 0 && lj < cpw && cs == (lj >> 2)) {
+      T part = 0;
+#pragma unroll
+      for (int i = 0; i < kTRPL; ++i) {
+        const int r = rg + 16 * i;
+        if (r <= j) part += x[TJ][i] * x[TJ][i];  // rows > j hold zeros
+      }
+      const T sigma2 = group16_sum1(part);
+      const T alpha = r1[TJ];
+      T beta, tau, scale;
+      if (sigma2 == T(0)) {
+        beta = alpha; tau = T(0); scale = T(0);
+      } else {
+        const T nrm = sqrt(alpha * alpha + sigma2);
+        beta = (alpha >= T(0)) ? -nrm : nrm;
+        tau = (beta - alpha) / beta;
+        scale = T(1) / (alpha - beta);
+      }
+#pragma unroll
+      for (int i = 0; i < kTRPL; ++i) {
+        const int r = rg + 16 * i;
+        x[TJ][i] = (r <= j) ? x[TJ][i] * scale : T(0);
+        vsh[buf][r] = x[TJ][i];
+      }
+      if (rg == 0) {
+        tau_sh[j] = tau;
+        r1s[j][j] = beta;  // read by no one before the end
+      }
+    }
+    __syncthreads();
+    const T tau = tau_sh[j];
+    // a wave whose columns all lie left of j + 1 has nothing to update
+    if (tau != T(0) && wave * cpw + cpw - 1 > j) {
+      T v[kTRPL];
+#pragma unroll
+      for (int i = 0; i < kTRPL; ++i) v[i] = vsh[buf][rg + 16 * i];
+      T part[kCPL];
+#pragma unroll
+      for (int t = 0; t < kCPL; ++t) {
+        part[t] = T(0);
+#pragma unroll
+        for (int i = 0; i < kTRPL; ++i) part[t] += v[i] * x[t][i];
+      }
+      group16_sum(part);
+#pragma unroll
+      for (int t = 0; t < kCPL; ++t) {
+        const int c = colof(t);
+        if (c > j) {
+          const T w = (r1[t] + part[t]) * tau;
+#pragma unroll
+          for (int i = 0; i < kTRPL; ++i) x[t][i] -= v[i] * w;
+          if (rg == 0) r1s[j][c] = r1[t] - w;  // row j of R
+        }
+      }
+    }
+  };
+  for (int j0 = 0; j0 < l; j0 += 4) {
+    tt_step(j0, std::integral_constant<int, 0>{});
+    if (j0 + 1 < l) tt_step(j0 + 1, std::integral_constant<int, 1>{});
+    if (j0 + 2 < l) tt_step(j0 + 2, std::integral_constant<int, 2>{});
+    if (j0 + 3 < l) tt_step(j0 + 3, std::integral_constant<int, 3>{});
+  }
+  __syncthreads();
+  // R (zeros below the diagonal), the taus, and the bottom parts of the reflectors over R2
+  // (rows > j of column j are zero)
+  for (int e = tid; e < l * l; e += kNT) {
+    const int r = e / l, c = e % l;
+    R[e] = (c < r) ? T(0) : r1s[r][c];
+  }
+  for (int e = tid; e < l; e += kNT) tau_out[e] = tau_sh[e];
+  if (pair) {
+#pragma unroll
+    for (int t = 0; t < kCPL; ++t) {
+      const int c = colof(t);
+      if (c < 0) continue;
+#pragma unroll
+      for (int i = 0; i < kTRPL; ++i) {
+        const int r = rg + 16 * i;
+        if (r < l) R2[r * l + c] = x[t][i];
+      }
     }
   }
 }
@@ -324,6 +462,121 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
   }
 }
 
+// Explicit Q at an internal node: [Xt; Xb] := H_0 ... H_{l-1} [C; 0] for the stacked pair
+// factored by tt_factor_kernel.  H_j touches top row j and bottom rows 0..j only, so top row
+// j is still C's row j when H_j is applied (last-to-first): it comes from C through LDS with
+// the reflectors, block by block, and leaves as the top child's row j.  The bottom child's
+// C is held in registers (8 rows per lane).  One workgroup barrier per block of kVB.
+template <typename T>
+__global__ __launch_bounds__(kNT) void tt_apply_kernel(LevelArgs a) {
+  constexpr int kRows = 16 * kTRPL;
+  constexpr int kLoad = kVB * kRows / kNT;  // entries of V and of C each thread loads per block
+  __shared__ T vsh[2][kVB][kRows + 1];
+  __shared__ T csh[2][kVB][kRows + 1];
+  const int j_node = blockIdx.x, b = blockIdx.y;
+  const int l = a.l, cpw = a.LC / kWaves;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int cs = lane >> 4, rg = lane & 15;
+  const int c0 = 2 * j_node;
+  const bool pair = c0 + 1 < a.nchild;
+  const T *C = reinterpret_cast<const T *>(a.Cin) + ((i64)b * a.nnode + j_node) * l * l;
+  T *Xt = reinterpret_cast<T *>(a.Cout) + ((i64)b * a.nchild + c0) * l * l;
+  if (!pair) {  // a lone child: every tau is zero, Q = I
+    for (int e = tid; e < l * l; e += kNT) Xt[e] = C[e];
+    return;
+  }
+  T *Xb = Xt + (i64)l * l;
+  const T *V2 = reinterpret_cast<const T *>(a.Rin) + ((i64)b * a.nchild + c0 + 1) * l * l;
+  const T *tau_in = reinterpret_cast<const T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
+  auto colof = [&](int t) {
+    const int lc = 4 * cs + t;
+    const int c = wave * cpw + lc;
+    return (lc < cpw && c < l) ? c : -1;
+  };
+  T x[kCPL][kTRPL];
+#pragma unroll
+  for (int t = 0; t < kCPL; ++t)
+#pragma unroll
+    for (int i = 0; i < kTRPL; ++i) x[t][i] = T(0);
+  // block j0: reflector parts v_j[r] = V2[r][j] (r <= j, else 0) with consecutive threads on
+  // consecutive j of one row, and C rows j (consecutive threads on consecutive columns)
+  T vreg[kLoad], creg[kLoad];
+  auto bload = [&](int j0) {
+#pragma unroll
+    for (int q = 0; q < kLoad; ++q) {
+      const int e = tid + kNT * q;
+      const int r = e / kVB, jj = e % kVB, j = j0 + jj;
+      vreg[q] = (j < l && r <= j && r < l) ? V2[r * l + j] : T(0);
+      const int jc = j0 + e / kRows, c = e % kRows;
+      creg[q] = (jc < l && c < l) ? C[jc * l + c] : T(0);
+    }
+  };
+  auto bstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < kLoad; ++q) {
+      const int e = tid + kNT * q;
+      vsh[buf][e % kVB][e / kVB] = vreg[q];
+      csh[buf][e / kRows][e % kRows] = creg[q];
+    }
+  };
+  const int nblk = (l + kVB - 1) / kVB;
+  bload((nblk - 1) * kVB);
+  bstore((nblk - 1) & 1);
+  for (int blk = nblk - 1; blk >= 0; --blk) {
+    const int buf = blk & 1, j0 = blk * kVB;
+    __syncthreads();  // block blk is in LDS; every wave is done with block blk + 1's buffers
+    if (blk > 0) bload(j0 - kVB);
+    const int jl = (j0 + kVB < l ? j0 + kVB : l) - 1;
+    for (int j = jl; j >= j0; --j) {
+      const T tau = tau_in[j];
+      T ct[kCPL];
+#pragma unroll
+      for (int t = 0; t < kCPL; ++t) {
+        const int c = colof(t);
+        ct[t] = (c >= 0) ? csh[buf][j - j0][c] : T(0);
+      }
+      if (tau != T(0)) {
+        T v[kTRPL];
+#pragma unroll
+        for (int i = 0; i < kTRPL; ++i) v[i] = vsh[buf][j - j0][rg + 16 * i];
+        T part[kCPL];
+#pragma unroll
+        for (int t = 0; t < kCPL; ++t) {
+          part[t] = T(0);
+#pragma unroll
+          for (int i = 0; i < kTRPL; ++i) part[t] += v[i] * x[t][i];
+        }
+        group16_sum(part);
+#pragma unroll
+        for (int t = 0; t < kCPL; ++t) {
+          const T w = (ct[t] + part[t]) * tau;
+          ct[t] -= w;
+#pragma unroll
+          for (int i = 0; i < kTRPL; ++i) x[t][i] -= v[i] * w;
+        }
+      }
+      if (rg == 0) {
+#pragma unroll
+        for (int t = 0; t < kCPL; ++t) {
+          const int c = colof(t);
+          if (c >= 0) Xt[j * l + c] = ct[t];
+        }
+      }
+    }
+    if (blk > 0) bstore(buf ^ 1);
+  }
+#pragma unroll
+  for (int t = 0; t < kCPL; ++t) {
+    const int c = colof(t);
+    if (c < 0) continue;
+#pragma unroll
+    for (int i = 0; i < kTRPL; ++i) {
+      const int r = rg + 16 * i;
+      if (r < l) Xb[r * l + c] = x[t][i];
+    }
+  }
+}
+
 template <typename T>
 __global__ void eye_rowmajor_kernel(int l, i64 count, T *C) {
   for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < count * l * l;
@@ -388,8 +641,12 @@ void tsqr_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr
     a.Rin = (k == 0) ? nullptr : Rbuf[k].get();
     a.tau = taus[k].get();
     a.Rout = Rbuf[k + 1].get();
-    hipLaunchKernelGGL(hh_factor_kernel<T>, dim3(nn[k], (unsigned)batch), dim3(kNT), 0,
-                       ctx().stream, a);
+    if (k == 0)
+      hipLaunchKernelGGL(hh_factor_kernel<T>, dim3(nn[k], (unsigned)batch), dim3(kNT), 0,
+                         ctx().stream, a);
+    else
+      hipLaunchKernelGGL(tt_factor_kernel<T>, dim3(nn[k], (unsigned)batch), dim3(kNT), 0,
+                         ctx().stream, a);
     LAUNCH_CHECK();
   }
   if (R) {
@@ -409,8 +666,12 @@ void tsqr_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr
     a.tau = taus[k].get();
     a.Cin = Cbuf[k + 1].get();
     a.Cout = (k == 0) ? nullptr : Cbuf[k].get();
-    hipLaunchKernelGGL(hh_apply_kernel<T>, dim3(nn[k], (unsigned)batch), dim3(kNT), 0,
-                       ctx().stream, a);
+    if (k == 0)
+      hipLaunchKernelGGL(hh_apply_kernel<T>, dim3(nn[k], (unsigned)batch), dim3(kNT), 0,
+                         ctx().stream, a);
+    else
+      hipLaunchKernelGGL(tt_apply_kernel<T>, dim3(nn[k], (unsigned)batch), dim3(kNT), 0,
+                         ctx().stream, a);
     LAUNCH_CHECK();
   }
 }
