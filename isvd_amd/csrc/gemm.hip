@@ -458,10 +458,17 @@ __global__ __launch_bounds__(dl::NT) void gemm_dl_kernel(GemmArgs<double> g) {
     // done (another wave refills that stage after the barrier)
     if (kt + S - 2 < nkf) __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(FILLS * (S - 2)));
     else __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(0));
+    // the deferred operands (instruction 1 of the last half step) are complete here: say so,
+    // or the compiler waits for them (and for every LDS read issued after the barrier) in
+    // front of the deferred products
+    asm volatile("" ::"v"(fa1[0][1]), "v"(fa1[1][1]), "v"(fa1[2][1]), "v"(fa1[3][1]),
+                 "v"(fb1[0][1]), "v"(fb1[1][1]), "v"(fb1[2][1]), "v"(fb1[3][1]));
     bare_barrier();
     if (kt + S - 1 < nkf)
       fill(kt + S - 1, stage(std::integral_constant<int, (kk + S - 1) % S>{}));
     load(stage(k), 0, fa0, fb0);
+    // keep the reads above the deferred products, which then cover their latency
+    __builtin_amdgcn_sched_barrier(0);
     mma(fa1, fb1, 1);
     mma(fa0, fb0, 0);
     mma(fa0, fb0, 1);
