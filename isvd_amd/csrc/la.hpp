@@ -67,9 +67,11 @@ void small_gemm(char ta, char tb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 
                 i64 ldb, T beta, T *C, i64 ldc, T diag = T(0));
 
 // C = (I/2 + (I/4 - Z)^{1/2})^{1/2} and C^-1 (n x n, ld n) by coupled Newton-Schulz square
-// roots; valid while ||4 Z||_2 < 1.  work: 6 n*n entries.
+// roots; valid while ||4 Z||_2 < 1.  work: 6 n*n entries.  err (device, may be null):
+// err[0] = ||C - I||_F.
 template <typename T>
-void kn_matfun(i64 n, const T *Z, T *C, T *Cinv, T *work, int iters1, int iters2);
+void kn_matfun(i64 n, const T *Z, T *C, T *Cinv, T *work, int iters1, int iters2,
+               T *err = nullptr);
 
 // out[0] = ||A - c I||_F  (n x n, ld n)
 template <typename T>

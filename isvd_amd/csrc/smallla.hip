@@ -88,16 +88,6 @@ void small_gemm2(int m, int n, int k, const SmallProblem<T> &p0, const SmallProb
   LAUNCH_CHECK();
 }
 
-// A := alpha * (A + A^T) / 2 + beta * I ; optionally B := gamma * I - delta * sym(A_in)
-template <typename T>
-__global__ void sym_affine_kernel(int n, const T *Z, T *out, T a, T b) {
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
-    const int i = e % n, j = e / n;
-    const T s = T(0.5) * (Z[i + j * n] + Z[j + i * n]);
-    out[e] = a * s + ((i == j) ? b : T(0));
-  }
-}
-
 // out[0] = ||A - c I||_F (one workgroup, fixed order)
 template <typename T>
 __global__ __launch_bounds__(1024) void frob_shift_kernel(int n, const T *A, T c, T *out) {
@@ -221,13 +211,49 @@ void small_gemm(char ta, char tb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 
 
 // C, Cinv (n x n, ld n) from Z; work must hold 6 n*n entries.  Returns nothing; the caller
 // decides from ||4 Z||_F whether this is valid.
+// Start of one Newton-Schulz stage: Y := a * sym(S) + b * I and W := I (one launch)
 template <typename T>
-void kn_matfun(i64 n, const T *Z, T *C, T *Cinv, T *work, int iters1, int iters2) {
+__global__ void ns_init_kernel(int n, const T *S, T *Y, T *W, T a, T b) {
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < n * n; e += gridDim.x * blockDim.x) {
+    const int i = e % n, j = e / n;
+    const T sy = T(0.5) * (S[i + j * n] + S[j + i * n]);
+    Y[e] = a * sy + ((i == j) ? b : T(0));
+    W[e] = (i == j) ? T(1) : T(0);
+  }
+}
+
+// End of the matrix functions: C := sym(Cr), Cinv := sym(Cir), and, when err is not null,
+// err[0] = ||C - I||_F (one workgroup, fixed reduction order)
+template <typename T>
+__global__ __launch_bounds__(1024) void ns_finish_kernel(int n, const T *Cr, const T *Cir, T *C,
+                                                         T *Cinv, T *err) {
+  __shared__ T red[16];
+  T part = 0;
+  for (int e = threadIdx.x; e < n * n; e += 1024) {
+    const int i = e % n, j = e / n;
+    const T c = T(0.5) * (Cr[i + j * n] + Cr[j + i * n]);
+    C[e] = c;
+    Cinv[e] = T(0.5) * (Cir[i + j * n] + Cir[j + i * n]);
+    const T d = c - ((i == j) ? T(1) : T(0));
+    part += d * d;
+  }
+  if (!err) return;
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    T t = 0;
+    for (int w = 0; w < 16; ++w) t += red[w];  // fixed order
+    err[0] = sqrt(t);
+  }
+}
+
+template <typename T>
+void kn_matfun(i64 n, const T *Z, T *C, T *Cinv, T *work, int iters1, int iters2, T *err) {
   const int nn = (int)(n * n);
   T *Y = work, *W = work + nn, *E = work + 2 * nn, *Y2 = work + 3 * nn, *W2 = work + 4 * nn;
+  // Y_0 and W_0 = I are set by the caller of ns
   auto ns = [&](T *y, T *w, T *y2, T *w2, int iters, T *&yo, T *&wo) {
-    // Y_0 = (set by caller), W_0 = I
-    fill<T>(n, n, T(0), w, n, T(1));
     for (int it = 0; it < iters; ++it) {
       // E = 3 I - W Y
       small_gemm<T>('N', 'N', n, n, n, T(-1), w, n, y, n, T(0), E, n, T(3));
@@ -242,26 +268,24 @@ void kn_matfun(i64 n, const T *Z, T *C, T *Cinv, T *work, int iters1, int iters2
     wo = w;
   };
   // stage 1: P = (I - 4 sym(Z))^{1/2} = 2 (I/4 - Z)^{1/2}
-  hipLaunchKernelGGL(sym_affine_kernel<T>, dim3(grid_of(nn)), dim3(256), 0, ctx().stream, (int)n,
-                     Z, Y, T(-4), T(1));
+  hipLaunchKernelGGL(ns_init_kernel<T>, dim3(grid_of(nn)), dim3(256), 0, ctx().stream, (int)n,
+                     Z, Y, W, T(-4), T(1));
   LAUNCH_CHECK();
   T *P, *Pi;
   ns(Y, W, Y2, W2, iters1, P, Pi);
   // stage 2: C^2 = I/2 + P/2 ; (C, C^-1) = NS square root
   T *Ys = (P == Y) ? Y2 : Y;
   T *Ws = (P == Y) ? W2 : W;
-  hipLaunchKernelGGL(sym_affine_kernel<T>, dim3(grid_of(nn)), dim3(256), 0, ctx().stream, (int)n,
-                     P, Ys, T(0.5), T(0.5));
+  hipLaunchKernelGGL(ns_init_kernel<T>, dim3(grid_of(nn)), dim3(256), 0, ctx().stream, (int)n,
+                     (const T *)P, Ys, Ws, T(0.5), T(0.5));
   LAUNCH_CHECK();
   T *Cr, *Cir;
   T *spare1 = (Ys == Y) ? Y2 : Y;
   T *spare2 = (Ws == W) ? W2 : W;
   ns(Ys, Ws, spare1, spare2, iters2, Cr, Cir);
-  // symmetrize the results
-  hipLaunchKernelGGL(sym_affine_kernel<T>, dim3(grid_of(nn)), dim3(256), 0, ctx().stream, (int)n,
-                     Cr, C, T(1), T(0));
-  hipLaunchKernelGGL(sym_affine_kernel<T>, dim3(grid_of(nn)), dim3(256), 0, ctx().stream, (int)n,
-                     Cir, Cinv, T(1), T(0));
+  // symmetrize the results (and ||C - I||_F)
+  hipLaunchKernelGGL(ns_finish_kernel<T>, dim3(1), dim3(1024), 0, ctx().stream, (int)n,
+                     (const T *)Cr, (const T *)Cir, C, Cinv, err);
   LAUNCH_CHECK();
 }
 
@@ -287,7 +311,7 @@ void inv_small(i64 n, T *A, i64 lda, int *info_dev) {
 #define INST(T)                                                                               \
   template void small_gemm<T>(char, char, i64, i64, i64, T, const T *, i64, const T *, i64, T, \
                               T *, i64, T);                                                    \
-  template void kn_matfun<T>(i64, const T *, T *, T *, T *, int, int);                         \
+  template void kn_matfun<T>(i64, const T *, T *, T *, T *, int, int, T *);                    \
   template void frob_shift<T>(i64, const T *, T, T *);                                         \
   template void inv_small<T>(i64, T *, i64, int *);
 INST(double)

@@ -339,8 +339,8 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
     int k1 = 0, k2 = 0;
     const bool use_eig = !ns_counts<T>(rho, &k1, &k2);
     if (!use_eig) {
-      kn_matfun<T>(l, Z, c.get(), cinv.get(), work.get(), k1, k2);
-      frob_shift<T>(l, c.get(), T(1), scal.get());  // ||I - C||_F = ||s - 1||_2
+      // and ||C - I||_F = ||s - 1||_2 into scal[0]
+      kn_matfun<T>(l, Z, c.get(), cinv.get(), work.get(), k1, k2, scal.get());
     } else {
       syev_small<T>(l, Z, l, w.get(), zv.get(), l);
       hipLaunchKernelGGL(kn_scalars_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
