@@ -87,6 +87,47 @@ __device__ __forceinline__ T group16_sum1(T v) {
   return row16_sum(v);
 }
 
+// The lane's columns c0 .. c0 + 3 (c0 = w * cpw + 4 cs) are consecutive, so a row of them is
+// read or written with one 4-element access when the group is whole and the row start is
+// 4-element aligned: the accesses of a wave then cover whole cache lines instead of 8 bytes
+// every 32 (which multiplied the HBM traffic of these kernels several times over).
+template <typename T> struct Quad;
+template <> struct Quad<double> { typedef double v __attribute__((ext_vector_type(4))); };
+template <> struct Quad<float> { typedef float v __attribute__((ext_vector_type(4))); };
+
+__device__ __forceinline__ int quad_cols(int cpw, int cs, int c0, int l) {
+  int n = cpw - 4 * cs;
+  if (l - c0 < n) n = l - c0;
+  return n < 0 ? 0 : (n > kCPL ? kCPL : n);
+}
+template <typename T>
+__device__ __forceinline__ bool quad_ok(const T *p, i64 ld, int ncol) {
+  return ncol == kCPL && (ld % kCPL) == 0 &&
+         (reinterpret_cast<uintptr_t>(p) % (kCPL * sizeof(T))) == 0;
+}
+template <typename T>
+__device__ __forceinline__ typename Quad<T>::v get_quad(const T *row, int ncol, bool vec) {
+  typename Quad<T>::v q = {T(0), T(0), T(0), T(0)};
+  if (vec) {
+    q = *reinterpret_cast<const typename Quad<T>::v *>(row);
+  } else {
+#pragma unroll
+    for (int t = 0; t < kCPL; ++t)
+      if (t < ncol) q[t] = row[t];
+  }
+  return q;
+}
+template <typename T>
+__device__ __forceinline__ void put_quad(T *row, int ncol, bool vec, typename Quad<T>::v q) {
+  if (vec) {
+    *reinterpret_cast<typename Quad<T>::v *>(row) = q;
+  } else {
+#pragma unroll
+    for (int t = 0; t < kCPL; ++t)
+      if (t < ncol) row[t] = q[t];
+  }
+}
+
 template <typename T>
 __device__ __forceinline__ void group16_sum(T (&v)[kCPL]) {
 #pragma unroll
@@ -116,15 +157,16 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
     const int c = wave * cpw + lc;
     return (lc < cpw && c < l) ? c : -1;
   };
+  const int c0 = wave * cpw + 4 * cs, ncol = quad_cols(cpw, cs, c0, l);
+  const bool vb = quad_ok(base, ld, ncol);
   T x[kCPL][kRPL];
 #pragma unroll
-  for (int t = 0; t < kCPL; ++t) {
-    const int c = colof(t);
+  for (int i = 0; i < kRPL; ++i) {
+    const int r = rg + 16 * i;
+    typename Quad<T>::v q = {T(0), T(0), T(0), T(0)};
+    if (r < h) q = get_quad(base + (i64)r * ld + c0, ncol, vb);
 #pragma unroll
-    for (int i = 0; i < kRPL; ++i) {
-      const int r = rg + 16 * i;
-      x[t][i] = (c >= 0 && r < h) ? base[(i64)r * ld + c] : T(0);
-    }
+    for (int t = 0; t < kCPL; ++t) x[t][i] = q[t];
   }
   const int kmax = h < l ? h : l;
   T *tau_out = reinterpret_cast<T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
@@ -198,16 +240,18 @@ __global__ __launch_bounds__(kNT) void hh_factor_kernel(LevelArgs a) {
   for (int e = tid; e < kmax; e += kNT) tau_out[e] = tau_sh[e];
   // V (below the diagonal) and R (on and above) back in place; R to the next level
   T *R = reinterpret_cast<T *>(a.Rout) + ((i64)b * a.nnode + j_node) * l * l;
+  const bool vr = quad_ok(R, l, ncol);
 #pragma unroll
-  for (int t = 0; t < kCPL; ++t) {
-    const int c = colof(t);
-    if (c < 0) continue;
+  for (int i = 0; i < kRPL; ++i) {
+    const int r = rg + 16 * i;
+    typename Quad<T>::v q, qr;
 #pragma unroll
-    for (int i = 0; i < kRPL; ++i) {
-      const int r = rg + 16 * i;
-      if (r < h) base[(i64)r * ld + c] = x[t][i];
-      if (r < l) R[r * l + c] = (r <= c && r < kmax) ? x[t][i] : T(0);
+    for (int t = 0; t < kCPL; ++t) {
+      q[t] = x[t][i];
+      qr[t] = (r <= c0 + t && r < kmax) ? x[t][i] : T(0);
     }
+    if (r < h) put_quad(base + (i64)r * ld + c0, ncol, vb, q);
+    if (r < l) put_quad(R + r * l + c0, ncol, vr, qr);
   }
 }
 
@@ -231,10 +275,10 @@ __global__ __launch_bounds__(kNT) void tt_factor_kernel(LevelArgs a) {
   const int l = a.l, cpw = a.LC / kWaves;
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int cs = lane >> 4, rg = lane & 15;
-  const int c0 = 2 * j_node;
-  const bool pair = c0 + 1 < a.nchild;  // a lone last child: R2 = 0
-  const T *R1 = reinterpret_cast<const T *>(a.Rin) + ((i64)b * a.nchild + c0) * l * l;
-  T *R2 = reinterpret_cast<T *>(a.Rin) + ((i64)b * a.nchild + c0 + 1) * l * l;
+  const int ch0 = 2 * j_node;  // first child
+  const bool pair = ch0 + 1 < a.nchild;  // a lone last child: R2 = 0
+  const T *R1 = reinterpret_cast<const T *>(a.Rin) + ((i64)b * a.nchild + ch0) * l * l;
+  T *R2 = reinterpret_cast<T *>(a.Rin) + ((i64)b * a.nchild + ch0 + 1) * l * l;
   T *tau_out = reinterpret_cast<T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
   T *R = reinterpret_cast<T *>(a.Rout) + ((i64)b * a.nnode + j_node) * l * l;
   auto colof = [&](int t) {
@@ -243,15 +287,16 @@ __global__ __launch_bounds__(kNT) void tt_factor_kernel(LevelArgs a) {
     return (lc < cpw && c < l) ? c : -1;
   };
   for (int e = tid; e < l * l; e += kNT) r1s[e / l][e % l] = R1[e];
+  const int c0 = wave * cpw + 4 * cs, ncol = quad_cols(cpw, cs, c0, l);
+  const bool v2 = quad_ok(R2, l, ncol);
   T x[kCPL][kTRPL];
 #pragma unroll
-  for (int t = 0; t < kCPL; ++t) {
-    const int c = colof(t);
+  for (int i = 0; i < kTRPL; ++i) {
+    const int r = rg + 16 * i;
+    typename Quad<T>::v q = {T(0), T(0), T(0), T(0)};
+    if (pair && r < l) q = get_quad(R2 + r * l + c0, ncol, v2);
 #pragma unroll
-    for (int i = 0; i < kTRPL; ++i) {
-      const int r = rg + 16 * i;
-      x[t][i] = (pair && c >= 0 && r < l) ? R2[r * l + c] : T(0);
-    }
+    for (int t = 0; t < kCPL; ++t) x[t][i] = q[t];
   }
   __syncthreads();
   auto tt_step = [&](int j, auto tj_c) __attribute__((always_inline)) {
@@ -337,14 +382,12 @@ __global__ __launch_bounds__(kNT) void tt_factor_kernel(LevelArgs a) {
   for (int e = tid; e < l; e += kNT) tau_out[e] = tau_sh[e];
   if (pair) {
 #pragma unroll
-    for (int t = 0; t < kCPL; ++t) {
-      const int c = colof(t);
-      if (c < 0) continue;
+    for (int i = 0; i < kTRPL; ++i) {
+      const int r = rg + 16 * i;
+      typename Quad<T>::v q;
 #pragma unroll
-      for (int i = 0; i < kTRPL; ++i) {
-        const int r = rg + 16 * i;
-        if (r < l) R2[r * l + c] = x[t][i];
-      }
+      for (int t = 0; t < kCPL; ++t) q[t] = x[t][i];
+      if (r < l) put_quad(R2 + r * l + c0, ncol, v2, q);
     }
   }
 }
@@ -377,14 +420,17 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
     const int c = wave * cpw + lc;
     return (lc < cpw && c < l) ? c : -1;
   };
+  const int c0 = wave * cpw + 4 * cs, ncol = quad_cols(cpw, cs, c0, l);
   T x[kCPL][kRPL];
-#pragma unroll
-  for (int t = 0; t < kCPL; ++t) {
-    const int c = colof(t);
+  {
+    const bool vc = quad_ok(C, l, ncol);
 #pragma unroll
     for (int i = 0; i < kRPL; ++i) {
       const int r = rg + 16 * i;
-      x[t][i] = (c >= 0 && r < kmax) ? C[r * l + c] : T(0);
+      typename Quad<T>::v q = {T(0), T(0), T(0), T(0)};
+      if (r < kmax) q = get_quad(C + r * l + c0, ncol, vc);
+#pragma unroll
+      for (int t = 0; t < kCPL; ++t) x[t][i] = q[t];
     }
   }
   // entry q of this thread's share of a block: row r = e / kVB, reflector jj = e % kVB
@@ -450,15 +496,14 @@ __global__ __launch_bounds__(kNT) void hh_apply_kernel(LevelArgs a) {
   }
   // every wave has read its V entries from the panel before any wave overwrites it with Q
   __syncthreads();
+  const bool vo = quad_ok(out, ldo, ncol);
 #pragma unroll
-  for (int t = 0; t < kCPL; ++t) {
-    const int c = colof(t);
-    if (c < 0) continue;
+  for (int i = 0; i < kRPL; ++i) {
+    const int r = rg + 16 * i;
+    typename Quad<T>::v q;
 #pragma unroll
-    for (int i = 0; i < kRPL; ++i) {
-      const int r = rg + 16 * i;
-      if (r < h) out[(i64)r * ldo + c] = x[t][i];
-    }
+    for (int t = 0; t < kCPL; ++t) q[t] = x[t][i];
+    if (r < h) put_quad(out + (i64)r * ldo + c0, ncol, vo, q);
   }
 }
 
@@ -477,16 +522,18 @@ __global__ __launch_bounds__(kNT) void tt_apply_kernel(LevelArgs a) {
   const int l = a.l, cpw = a.LC / kWaves;
   const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
   const int cs = lane >> 4, rg = lane & 15;
-  const int c0 = 2 * j_node;
-  const bool pair = c0 + 1 < a.nchild;
+  const int ch0 = 2 * j_node;  // first child
+  const bool pair = ch0 + 1 < a.nchild;
   const T *C = reinterpret_cast<const T *>(a.Cin) + ((i64)b * a.nnode + j_node) * l * l;
-  T *Xt = reinterpret_cast<T *>(a.Cout) + ((i64)b * a.nchild + c0) * l * l;
+  T *Xt = reinterpret_cast<T *>(a.Cout) + ((i64)b * a.nchild + ch0) * l * l;
   if (!pair) {  // a lone child: every tau is zero, Q = I
     for (int e = tid; e < l * l; e += kNT) Xt[e] = C[e];
     return;
   }
   T *Xb = Xt + (i64)l * l;
-  const T *V2 = reinterpret_cast<const T *>(a.Rin) + ((i64)b * a.nchild + c0 + 1) * l * l;
+  const int c0 = wave * cpw + 4 * cs, ncol = quad_cols(cpw, cs, c0, l);
+  const bool vq = quad_ok(Xt, l, ncol);  // Xt and Xb: same alignment, leading dimension l
+  const T *V2 = reinterpret_cast<const T *>(a.Rin) + ((i64)b * a.nchild + ch0 + 1) * l * l;
   const T *tau_in = reinterpret_cast<const T *>(a.tau) + ((i64)b * a.nnode + j_node) * l;
   auto colof = [&](int t) {
     const int lc = 4 * cs + t;
@@ -556,24 +603,21 @@ __global__ __launch_bounds__(kNT) void tt_apply_kernel(LevelArgs a) {
         }
       }
       if (rg == 0) {
+        typename Quad<T>::v q;
 #pragma unroll
-        for (int t = 0; t < kCPL; ++t) {
-          const int c = colof(t);
-          if (c >= 0) Xt[j * l + c] = ct[t];
-        }
+        for (int t = 0; t < kCPL; ++t) q[t] = ct[t];
+        put_quad(Xt + j * l + c0, ncol, vq, q);
       }
     }
     if (blk > 0) bstore(buf ^ 1);
   }
 #pragma unroll
-  for (int t = 0; t < kCPL; ++t) {
-    const int c = colof(t);
-    if (c < 0) continue;
+  for (int i = 0; i < kTRPL; ++i) {
+    const int r = rg + 16 * i;
+    typename Quad<T>::v q;
 #pragma unroll
-    for (int i = 0; i < kTRPL; ++i) {
-      const int r = rg + 16 * i;
-      if (r < l) Xb[r * l + c] = x[t][i];
-    }
+    for (int t = 0; t < kCPL; ++t) q[t] = x[t][i];
+    if (r < l) put_quad(Xb + r * l + c0, ncol, vq, q);
   }
 }
 
