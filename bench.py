@@ -6,7 +6,7 @@ i < 2048, U / V seeded orthonormal (CholeskyQR2 of Gaussian blocks on the GPU), 
 resident in HBM before the timed region; one step is one full decomposition through the C ABI
 (isvd_dIsvdOmega).  Wen-Yin integration is timed separately on the same input.
 
-Prints ONE JSON line (rank 0).  --gpus N > 1 is rejected: this build runs on one GPU.
+Prints ONE JSON line (rank 0).  --gpus N > 1 is rejected: the library runs on one GPU.
 """
 import argparse
 import json
@@ -252,7 +252,7 @@ def main():
                     help="run one sketch GEMM and one 1 GiB device copy (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this iSVD build runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "the iSVD library runs on one GPU; --gpus must be 1"}))
         sys.exit(2)
 
     import numpy as np
