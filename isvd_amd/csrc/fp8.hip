@@ -149,10 +149,12 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef __attribute__((address_space(3))) void lds_void_t;
 
 // 16-byte chunk swizzle of a 64-byte LDS row: chunk c of row r sits at position
-// c ^ swz(r).  Among rows 0..7, and among the rows of each residue mod 4 in 0..15, the
-// positions differ, so a fragment read (consecutive rows, one chunk each) hits distinct
-// banks whether the LDS serves 8 or 16 lanes of a ds_read_b128 per pass.
-__device__ inline int swz(int r) { return ((r >> 1) ^ (r >> 3)) & 3; }
+// c ^ swz(r), with swz(r) = (r / 4) mod 4.  Chosen by measurement on gfx950: for the fragment
+// read (lane l: row l % 32, chunk 2 (l / 32) + h, one ds_read_b128) this swizzle, like
+// (r / 8) mod 4, counts no LDS bank conflicts and runs as fast as a contiguous read, while
+// ((r / 2) ^ (r / 8)) mod 4, (r / 2) mod 4 and ((r / 4) ^ (r / 16)) mod 4 count 4 conflict
+// cycles per instruction (1.6x the time) and r mod 4 or no swizzle 12 (3x).
+__device__ inline int swz(int r) { return (r >> 2) & 3; }
 __device__ inline int lds_off(int row, int c) { return row * 64 + ((c ^ swz(row)) << 4); }
 
 // Block tile 256 (Omega rows) x 256 (A rows), four waves in a 2 x 2 grid, 128 x 128 per wave
