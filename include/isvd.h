@@ -326,12 +326,15 @@ void isvd_bPostprocessGramian(const isvd_Param param, const float *argv, const i
  * (e4m3fn: bias 7, largest finite value 448, 0x7F / 0xFF NaN) with one FP32 scale per
  * 128 x 128 block:  A[i, j] = e4m3(a8[i * lda8 + j]) * scale[(i / 128) * ldscale + j / 128].
  *   a8      row-major m x n (the n index contiguous), lda8 >= n, lda8 % 16 == 0, 16-byte
- *           aligned;
+ *           aligned; rows are read and written in whole 16-byte chunks, so the buffer must
+ *           span (m - 1) * lda8 + roundup(n, 16) bytes (m * lda8 always suffices);
  *   scale   ceil(m / 128) x ceil(n / 128), row-major, ldscale >= ceil(n / 128); every scale
  *           a power of two in [2^-127, 2^127] (applied on the matrix cores as an exact E8M0
  *           exponent; any other value fails the call);
  *   omega   Omega (n x Nl) as unscaled e4m3 codes, column c at omega + c * ldomega (the n
- *           index contiguous), ldomega >= n, ldomega % 16 == 0.
+ *           index contiguous), ldomega >= n, ldomega % 16 == 0; spans
+ *           (Nl - 1) * ldomega + roundup(n, 16) bytes (Nl * ldomega always suffices), and the
+ *           codes of rows n .. roundup(n, 16) - 1 must be zero.
  * The seeded forms draw Omega from the FP32 stream of isvd_sIsvd, rounded to e4m3.
  * isvd_eIsvd sketches from (a8, scale) on FP8 matrix cores (FP32 accumulation) and runs the
  * remaining stages as isvd_bIsvd on the BF16 copy a (orthogonalization, integration, and the

@@ -43,6 +43,7 @@ void fail(const std::string &msg) {
 static Ctx g_ctx;
 static bool g_init = false;
 static std::mutex g_mutex;
+static uint64_t g_pool_threshold = 0;  // the default pool's release threshold before init
 
 Ctx &ctx() { return g_ctx; }
 
@@ -65,6 +66,7 @@ void ensure_init() {
   // next allocation maps it again (milliseconds).  isvd_Finalize_gpu trims the pool.
   hipMemPool_t pool = nullptr;
   HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, 0));
+  HIP_CHECK(hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &g_pool_threshold));
   uint64_t keep = UINT64_MAX;
   HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
   g_init = true;
@@ -72,10 +74,32 @@ void ensure_init() {
 
 void sync() { HIP_CHECK(hipStreamSynchronize(g_ctx.stream)); }
 
+void *Arena::take(size_t bytes) {
+  const size_t at = (used + 255) / 256 * 256;
+  ISVD_CHECK(at <= cap && bytes <= cap - at,
+             "the %zu-byte scratch region is exhausted (%zu bytes requested, %zu in use)", cap,
+             bytes, at);
+  used = at + bytes;
+  return base + at;
+}
+
 static void *g_ws = nullptr;
 static size_t g_ws_bytes = 0;
 
+size_t splitk_budget() {
+  if (!g_ctx.arena) return SIZE_MAX;
+  const size_t at = (g_ctx.arena->used + 255) / 256 * 256;
+  return at < g_ctx.arena->cap ? g_ctx.arena->cap - at : 0;
+}
+
 void *splitk_workspace(size_t bytes) {
+  if (g_ctx.arena) {
+    // the partials of one product live at the top of the arena, above every buffer taken so
+    // far; later buffers may reuse that space, since the stream orders them after the product
+    ISVD_CHECK(bytes <= splitk_budget(), "split-K partials (%zu bytes) exceed the scratch left",
+               bytes);
+    return g_ctx.arena->base + (g_ctx.arena->cap - bytes) / 256 * 256;
+  }
   if (bytes > g_ws_bytes) {
     if (g_ws) HIP_CHECK(hipFreeAsync(g_ws, g_ctx.stream));
     g_ws = nullptr;
@@ -147,6 +171,8 @@ struct Arg {
 
 // elements spanned by an m x n column-major matrix with leading dimension ld
 inline i64 span(i64 m, i64 n, i64 ld) { return (m <= 0 || n <= 0) ? 0 : (n - 1) * ld + m; }
+// e4m3 rows are read and written in whole 16-byte chunks: a row of n codes spans roundup(n, 16)
+inline i64 round16(i64 n) { return (n + 15) / 16 * 16; }
 
 // ---------------------------------------------------------------------------------------------
 // argument parsing: isvd_arg2char / isvd_arg2str (src/lib/libisvd/util/arg.h:22-79)
@@ -600,7 +626,7 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
     if (e8) {
       // e4m3 A (row-major) and its block scales; Omega drawn as e4m3 with K contiguous
       const i64 nbi = (d.m + kF8Block - 1) / kF8Block, nbj = (d.n + kF8Block - 1) / kF8Block;
-      Arg<uint8_t> A8(e8->a8, span(d.n, d.m, e8->lda8), true, false);
+      Arg<uint8_t> A8(e8->a8, span(round16(d.n), d.m, e8->lda8), true, false);
       Arg<float> SC(e8->scale, span(nbj, nbi, e8->lds), true, false);
       const i64 ldo8 = (d.n + 15) / 16 * 16;
       DBuf<uint8_t> om8(ldo8 * d.Nl);
@@ -742,10 +768,10 @@ static void sketch_e_impl(const isvd_Param &param, char dista, const uint8_t *a8
                "ldomega (%lld) must be >= n and a multiple of 16", ldo);
   ISVD_CHECK(a8 != nullptr && scale != nullptr && yst != nullptr, "NULL argument");
   ensure_init();
-  Arg<uint8_t> A8(a8, span(d.n, d.m, lda8), true, false);
+  Arg<uint8_t> A8(a8, span(round16(d.n), d.m, lda8), true, false);
   Arg<float> SC(scale, span(nbj, nbi, lds), true, false);
   Arg<float> Y(yst, span(d.Nl, d.m, ldyst), false, true);
-  Arg<uint8_t> O(use_seed ? nullptr : omega8, use_seed ? 0 : span(d.n, d.Nl, ldo), true, false);
+  Arg<uint8_t> O(use_seed ? nullptr : omega8, use_seed ? 0 : span(round16(d.n), d.Nl, ldo), true, false);
   DBuf<uint8_t> om;
   const uint8_t *omd = O.dev;
   if (use_seed) {
@@ -769,7 +795,7 @@ static void quantize_impl(char ordera, i64 m, i64 n, const float *a, i64 lda, ui
   const i64 nbi = (m + kF8Block - 1) / kF8Block, nbj = (n + kF8Block - 1) / kF8Block;
   ISVD_CHECK(lds >= nbj, "ldscale (%lld) must be >= ceil(n / 128) (%lld)", lds, nbj);
   Arg<float> A(a, ordera_ == 'C' ? span(m, n, lda) : span(n, m, lda), true, false);
-  Arg<uint8_t> A8(a8, span(n, m, lda8), false, true);
+  Arg<uint8_t> A8(a8, span(round16(n), m, lda8), false, true);
   Arg<float> SC(scale, span(nbj, nbi, lds), false, true);
   quantize_e4m3(ordera_, m, n, A.dev, lda, A8.dev, lda8, SC.dev, lds);
   A8.finish();
@@ -831,6 +857,7 @@ void isvd_Finalize_gpu(void) {
       hipMemPool_t pool = nullptr;
       HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, 0));
       HIP_CHECK(hipMemPoolTrimTo(pool, 0));
+      HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &g_pool_threshold));
       g_init = false;
     }
   });

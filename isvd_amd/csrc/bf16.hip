@@ -1,6 +1,6 @@
 // BF16 conversions for the mixed-precision pipeline: round-to-nearest-even BF16 of FP32 data,
 // the materialized BF16 splits x = h0 + h1 (+ h2) that turn the compensated products into
-// plain BF16 GEMMs (hipBLASLt, blaslt.hip), and the small kernels that combine their results.
+// plain BF16 GEMMs (bf16_gemm.hip), and the small kernels that combine their results.
 #include "mx.hpp"
 
 namespace isvd {

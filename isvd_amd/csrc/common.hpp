@@ -56,46 +56,74 @@ void set_last_error(const std::string &msg);
 // Execution context: one HIP stream per library call (default priority), stream-ordered
 // allocations from the device's default memory pool.
 // ---------------------------------------------------------------------------------------------
+// A fixed device region handed out front to back (256-byte aligned pieces) in place of
+// stream-ordered allocations, for entry points that must not allocate.  Nothing is returned
+// to it during a call; the call that installed it resets it.
+struct Arena {
+  unsigned char *base = nullptr;
+  size_t cap = 0;
+  size_t used = 0;
+  void *take(size_t bytes);          // fails with an error when the region is exhausted
+  size_t left() const { return cap - used; }
+};
+
 struct Ctx {
   hipStream_t stream = nullptr;
   int device = 0;
+  Arena *arena = nullptr;  // non-null while a no-allocation entry point runs
 };
 
 Ctx &ctx();                // library context (created by isvd_Init_gpu or lazily)
 void ensure_init();         // selects device 0, creates the stream
 
-// Device buffer owned by the caller's scope, allocated stream-ordered.
+// Device buffer owned by the caller's scope, allocated stream-ordered (or taken from the
+// active arena, which needs no release).
 template <typename T>
 struct DBuf {
   T *p = nullptr;
   i64 n = 0;
+  bool arena = false;
   DBuf() = default;
   explicit DBuf(i64 count) { alloc(count); }
   void alloc(i64 count) {
     release();
     n = count;
     if (count > 0) {
-      HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&p), sizeof(T) * (size_t)count,
-                               ctx().stream));
+      if (ctx().arena) {
+        p = static_cast<T *>(ctx().arena->take(sizeof(T) * (size_t)count));
+        arena = true;
+      } else {
+        HIP_CHECK(hipMallocAsync(reinterpret_cast<void **>(&p), sizeof(T) * (size_t)count,
+                                 ctx().stream));
+      }
     }
   }
   void release() {
     if (p) {
-      (void)hipFreeAsync(p, ctx().stream);
+      if (!arena) (void)hipFreeAsync(p, ctx().stream);
       p = nullptr;
+      arena = false;
     }
     n = 0;
   }
   ~DBuf() { release(); }
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
-  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n), arena(o.arena) {
+    o.p = nullptr;
+    o.n = 0;
+    o.arena = false;
+  }
   T *get() const { return p; }
 };
 
 // Split-K partial sums: one device buffer per library context, grown on demand and reused by
 // every product (all work is ordered on ctx().stream, so consecutive products never overlap).
+// Under an arena the partials are taken from it instead.
 void *splitk_workspace(size_t bytes);
+// Bytes the split-K partials of one product may occupy (the arena's free space under an arena,
+// unbounded otherwise).
+size_t splitk_budget();
 
 // Host <- device copy of a few values that returns as soon as they land: the copy goes to
 // pinned host memory and the host spins on an event instead of a blocking stream wait.

@@ -602,6 +602,12 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
       ksplit = best;
     }
   }
+  // the partial sums must fit the space available for them (bounded under a scratch arena)
+  {
+    const size_t per = sizeof(T) * (size_t)(batch * g.m * g.n);
+    const size_t budget = splitk_budget();
+    while (ksplit > 1 && per * (size_t)ksplit > budget) --ksplit;
+  }
   i64 kchunk = (g.k + ksplit - 1) / ksplit;
   kchunk = (kchunk + BK - 1) / BK * BK;
   ksplit = (int)((g.k + kchunk - 1) / kchunk);

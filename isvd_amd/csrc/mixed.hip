@@ -25,14 +25,14 @@ int grid_for(i64 n) {
 
 // M := Qs^T Qs (Nl x Nl) from the FP32 bases by split-BF16 x3 products: Qs = H + L (two BF16
 // terms), M = H^T H + L^T H + H^T L.  The split is materialized as S = [H^T ; L^T] (2Nl x m)
-// and the two products [H^T H ; L^T H] = S H are one plain BF16 GEMM (hipBLASLt, FP32
-// accumulation); the combination symmetrizes exactly.
+// and the two products [H^T H ; L^T H] = S H are one plain BF16 GEMM (FP32 accumulation); the
+// combination symmetrizes exactly.
 void gram_split(const Dims &d, const float *yst, i64 ldyst, float *m32) {
   const i64 Nl = d.Nl, m = d.m;
   DBuf<uint16_t> S(2 * Nl * m);
   split_bf16(Nl, m, 2, yst, ldyst, S.get(), 2 * Nl);
   DBuf<float> C(2 * Nl * Nl);
-  gemm_bf16_lib('N', 'T', 2 * Nl, Nl, m, 1.f, S.get(), 2 * Nl, S.get(), 2 * Nl, 0.f, C.get(),
+  gemm_bf16('N', 'T', 2 * Nl, Nl, m, 1.f, S.get(), 2 * Nl, S.get(), 2 * Nl, 0.f, C.get(),
                 2 * Nl);
   S.release();
   gram_combine(Nl, C.get(), 2 * Nl, m32, Nl);
@@ -52,7 +52,7 @@ void gaussian_omega_b(i64 rows, i64 cols, uint16_t *omegat, i64 ld, unsigned lon
 void stage_sketch_b(const Dims &d, char ordera, const uint16_t *a, i64 lda,
                     const uint16_t *omegat, i64 ldo, float *yst, i64 ldyst) {
   // op(A)^T: a column-major A (m x n) is transposed, a row-major A is already A^T (n x m)
-  gemm_bf16_lib('N', ordera == 'C' ? 'T' : 'N', d.Nl, d.m, d.n, 1.f, omegat, ldo, a, lda, 0.f,
+  gemm_bf16('N', ordera == 'C' ? 'T' : 'N', d.Nl, d.m, d.n, 1.f, omegat, ldo, a, lda, 0.f,
                 yst, ldyst);
 }
 
@@ -127,7 +127,7 @@ void stage_post_b(const Dims &d, bool tsqr, char ordera, const uint16_t *a, i64 
     DBuf<uint16_t> q3(3 * l * m);
     split_bf16(l, m, 3, qt, ldqt, q3.get(), 3 * l);
     DBuf<float> c3(3 * l * n);
-    gemm_bf16_lib('N', ordera == 'C' ? 'N' : 'T', 3 * l, n, m, 1.f, q3.get(), 3 * l, a, lda,
+    gemm_bf16('N', ordera == 'C' ? 'N' : 'T', 3 * l, n, m, 1.f, q3.get(), 3 * l, a, lda,
                   0.f, c3.get(), 3 * l);
     sum_blocks(l, n, 3, c3.get(), 3 * l, zt32.get(), l);
   }

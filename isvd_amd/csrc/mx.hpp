@@ -1,5 +1,5 @@
-// BF16 products with FP32 accumulation for the mixed-precision pipeline (hipBLASLt), plus the
-// precision conversions around them.  FP32 data enter a product as materialized BF16 splits
+// BF16 products with FP32 accumulation for the mixed-precision pipeline (the hand-written
+// matrix-core kernel of bf16_gemm.hip), plus the precision conversions around them.  FP32 data enter a product as materialized BF16 splits
 // x = h0 + h1 (+ h2), each term the round-to-nearest-even BF16 of the remainder: two terms for
 // the "split bf16 x3" Gram (hi*hi + hi*lo + lo*hi), three terms (exact to FP32) against exact
 // BF16 data for the projection.
@@ -10,9 +10,9 @@
 namespace isvd {
 
 // C (m x n, FP32, ldc) := alpha op(A) op(B) + beta C for BF16 A, B (column-major, op 'N' / 'T')
-// through hipBLASLt
-void gemm_bf16_lib(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t *A,
-                   i64 lda, const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc);
+// on the BF16 matrix cores with FP32 accumulation
+void gemm_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t *A, i64 lda,
+               const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc);
 
 // dst (rows x cols, ldd) := src (rows x cols, lds), converted
 template <typename Tout, typename Tin>

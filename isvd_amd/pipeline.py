@@ -172,7 +172,6 @@ def isvd_backward(param, a, qbar, gs=None, gu=None, gv=None, alg_p="TS", ordera=
     (m x l), ``gs`` (k), ``gu`` (m x k) and ``gv`` (n x k) the gradients of the outputs (None for
     zero).  Returns dL/dA with the shape and storage of ``a``."""
     lib = capi.load()
-    _ready(a, qbar, gs, gu, gv)
     x, dt = _family(a)
     if x == "b":
         raise ValueError("isvd_backward: the mixed-precision family has no adjoint")
@@ -184,7 +183,9 @@ def isvd_backward(param, a, qbar, gs=None, gu=None, gv=None, alg_p="TS", ordera=
             return None
         return t.contiguous() if _is_torch(t) else np.ascontiguousarray(t)
 
+    # the contiguous copies are queued on the caller's stream: wait for them too
     gs, gu, gv = cont(gs), cont(gu), cont(gv)
+    _ready(a, qbar, gs, gu, gv)
     getattr(lib, f"isvd_{x}IsvdBackward")(
         param, alg_p.encode(), capi._chr(dista), capi._chr(ordera), capi.addr(a),
         _a_ld(a, ordera), capi.addr(qbar), qbar.shape[1], capi.addr(gs), capi.addr(gu), k,

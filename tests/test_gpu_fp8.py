@@ -179,6 +179,8 @@ def test_host_arrays_are_staged_like_device_arrays(torch_cuda):
     a8_h, sc_h = pipeline.quantize_e4m3(a)
     a8_d, sc_d = pipeline.quantize_e4m3(f32_dev(torch_cuda, a))
     assert np.array_equal(a8_h[:, :n], a8_d.cpu().numpy()[:, :n])
+    # whole 16-byte chunks come back, so the zero codes past column n reach the host buffer
+    assert a8_h.shape[1] == 304 and not a8_h[:, n:].any()
     assert np.array_equal(sc_h, sc_d.cpu().numpy())
     p = capi.create_param(m, n, l, 0, N)
     y_h = pipeline.sketch_e(p, a8_h, sc_h, seed=3)
