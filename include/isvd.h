@@ -234,6 +234,31 @@ ISVD_DECLARE_STAGES(s, isvd_s_val_t)
 ISVD_DECLARE_STAGES(d, isvd_d_val_t)
 
 /* =========================================================================================
+ * Gramian postprocessing that allocates nothing (no counterpart in the reference; for callers
+ * that cannot allocate device memory: tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments, checks and results as
+ * isvd_dPostprocessGramian, except:
+ *   - a, qt, s, ut and vt must be device memory (host arrays are rejected, since staging them
+ *     would allocate);
+ *   - every temporary (Z^T, the l x l Gram and its singular vectors, split-K partial sums)
+ *     lives in a fixed 64 MiB region of device memory per GPU that the library holds until
+ *     the process ends; a problem with 8 (l n + 2 l^2) bytes > 64 MiB is rejected with an
+ *     error (l = 128: n <= 65280).  When the split-K partial sums of a product do not fit
+ *     beside those, the product uses fewer K chunks (results then differ by FP64 rounding);
+ *   - calls on one GPU share the region and keep the last call's contents; calls that
+ *     overlapped on one GPU would race.  Within a process the library lock serializes them
+ *     and each call drains its stream before returning.
+ * ========================================================================================= */
+void isvd_dPostprocessGramianNoAlloc(const isvd_Param param, const isvd_d_val_t *argv,
+                                     const isvd_int_t argc, isvd_d_val_t *retv,
+                                     const isvd_int_t retc, const char dista, const char ordera,
+                                     const isvd_d_val_t *a, const isvd_int_t lda,
+                                     const isvd_d_val_t *qt, const isvd_int_t ldqt,
+                                     isvd_d_val_t *s, isvd_d_val_t *ut, const isvd_int_t ldut,
+                                     isvd_d_val_t *vt, const isvd_int_t ldvt,
+                                     const mpi_int_t ut_root, const mpi_int_t vt_root);
+
+/* =========================================================================================
  * Adjoint of the driver's rank-k result (no counterpart in the reference; for callers that
  * differentiate through the decomposition, e.g. a torch.autograd.Function).
  * Given dL/dS (gs, k values), dL/dU^T (gut, k x m, ld >= k) and dL/dV^T (gvt, k x n,

@@ -99,6 +99,8 @@ def load(path=None):
         _sig(getattr(lib, f"isvd_{x}PostprocessSymmetric_gpu"), post)
         _sig(getattr(lib, f"isvd_{x}IsvdBackward"),
              [Param, ctypes.c_char_p, CH, CH, VP, I, VP, I, VP, VP, I, VP, I, VP, I])
+    _sig(lib.isvd_dPostprocessGramianNoAlloc,
+         [Param, VP, I, VP, I, CH, CH, VP, I, VP, I, VP, VP, I, VP, I, mpi_int_t, mpi_int_t])
     # e family: FP8 (e4m3) sketch
     _sig(lib.isvd_eQuantize, [CH, I, I, VP, I, VP, I, VP, I])
     _sig(lib.isvd_eSketchGaussianProjection,

@@ -45,6 +45,15 @@ static bool g_init = false;
 static std::mutex g_mutex;
 static uint64_t g_pool_threshold = 0;  // the default pool's release threshold before init
 
+// Scratch region of isvd_dPostprocessGramianNoAlloc: every temporary of that entry point
+// (Z^T, the l x l Gram and its singular vectors, split-K partial sums) is carved from it, so
+// the call allocates nothing.  Each GPU the library runs on holds one copy for the life of the
+// process.  In host code `scratch` names a host-side stand-in: kernels and copies use the
+// device address recorded at initialization (g_scratch_dev), never the symbol itself.
+constexpr size_t kScratchBytes = 64ull << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static unsigned char *g_scratch_dev = nullptr;
+
 Ctx &ctx() { return g_ctx; }
 
 void ensure_init() {
@@ -69,6 +78,10 @@ void ensure_init() {
   HIP_CHECK(hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &g_pool_threshold));
   uint64_t keep = UINT64_MAX;
   HIP_CHECK(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
+  // device address of this GPU's copy of the no-allocation scratch region
+  void *sp = nullptr;
+  HIP_CHECK(hipGetSymbolAddress(&sp, HIP_SYMBOL(scratch)));
+  g_scratch_dev = static_cast<unsigned char *>(sp);
   g_init = true;
 }
 
@@ -485,6 +498,57 @@ static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera
   S.finish();
   U.finish();
   V.finish();
+  sync();
+}
+
+// Gramian postprocessing without allocation: the checks and stages of post_impl (kind 'G', FP64)
+// with every temporary carved from this GPU's scratch region.  Device pointers only (staging
+// host arrays would allocate).  Z^T, W and U_W are taken from the bottom of the region, each
+// product's split-K partial sums from the space above them (fewer K chunks when that space is
+// short), so the footprint is 8 (l n + 2 l^2) bytes plus alignment.  Nothing in the region needs
+// to start at zero: every kernel writes what it later reads.
+static void post_gramian_noalloc_impl(const isvd_Param &param, i64 argc, char dista, char ordera,
+                                      const double *a, i64 lda, const double *qt, i64 ldqt,
+                                      double *s, double *ut, i64 ldut, double *vt, i64 ldvt,
+                                      int ut_root, int vt_root) {
+  if (argc < 0) return;
+  const char dista_ = arg2char("DISTA", dista, "CR");
+  const char ordera_ = arg2char("ORDERA", ordera, "CR");
+  if (!dista_ || !ordera_) return;
+  check_param(param);
+  const Dims d = dims_of(param);
+  if (ut_root >= 0) ISVD_CHECK(ldut == d.l, "ldut (%lld) must equal l (%lld)", ldut, d.l);
+  else if (ut_root == -1) ISVD_CHECK(ldut >= d.l, "ldut (%lld) must be >= l", ldut);
+  if (vt_root >= 0) ISVD_CHECK(ldvt == d.l, "ldvt (%lld) must equal l (%lld)", ldvt, d.l);
+  else if (vt_root == -1) ISVD_CHECK(ldvt >= d.l, "ldvt (%lld) must be >= l", ldvt);
+  ISVD_CHECK(lda >= (ordera_ == 'C' ? d.m : d.n), "lda (%lld) too small", lda);
+  ISVD_CHECK(ldqt >= d.l, "ldqt (%lld) must be >= l", ldqt);
+  ISVD_CHECK(d.l <= 128, "l = k + p must be <= 128 (got %lld)", d.l);
+  const bool want_u = ut_root >= -1, want_v = vt_root >= -1;
+  const i64 need = (d.l * d.n * 8 + 255) / 256 * 256 + 2 * ((d.l * d.l * 8 + 255) / 256 * 256);
+  ISVD_CHECK(need <= (i64)kScratchBytes,
+             "Z^T, W and U_W need %lld bytes (8 (l n + 2 l^2)), more than the %zu-byte scratch "
+             "region (l = %lld, n = %lld)",
+             need, kScratchBytes, d.l, d.n);
+  ensure_init();
+  ISVD_CHECK(g_scratch_dev != nullptr, "the scratch region has no device address");
+  auto dev = [](const void *p, const char *name) {
+    ISVD_CHECK(p != nullptr && is_device_pointer(p),
+               "%s must be device memory (staging a host array would allocate)", name);
+  };
+  dev(a, "a");
+  dev(qt, "qt");
+  dev(s, "s");
+  if (want_u) dev(ut, "ut");
+  if (want_v) dev(vt, "vt");
+  Arena arena;
+  arena.base = g_scratch_dev;
+  arena.cap = kScratchBytes;
+  struct Install {
+    explicit Install(Arena *a) { g_ctx.arena = a; }
+    ~Install() { g_ctx.arena = nullptr; }
+  } install(&arena);
+  stage_post_gramian<double>(d, ordera_, a, lda, qt, ldqt, s, ut, ldut, vt, ldvt, want_u, want_v);
   sync();
 }
 
@@ -1055,6 +1119,21 @@ void isvd_Free(void *ptr) { std::free(ptr); }
                                  ldqt, s, ut, ldut, vt, ldvt, ut_root, vt_root);              \
   }
 
+
+void isvd_dPostprocessGramianNoAlloc(const isvd_Param param, const isvd_d_val_t *argv,
+                                     const isvd_int_t argc, isvd_d_val_t *retv,
+                                     const isvd_int_t retc, const char dista, const char ordera,
+                                     const isvd_d_val_t *a, const isvd_int_t lda,
+                                     const isvd_d_val_t *qt, const isvd_int_t ldqt,
+                                     isvd_d_val_t *s, isvd_d_val_t *ut, const isvd_int_t ldut,
+                                     isvd_d_val_t *vt, const isvd_int_t ldvt,
+                                     const mpi_int_t ut_root, const mpi_int_t vt_root) {
+  (void)argv; (void)retv; (void)retc;
+  guarded([&] {
+    post_gramian_noalloc_impl(param, argc, dista, ordera, a, lda, qt, ldqt, s, ut, ldut, vt,
+                              ldvt, ut_root, vt_root);
+  });
+}
 
 ISVD_DEFINE(s, PrecS, isvd_s_val_t, isvd_s_val_t)
 ISVD_DEFINE(d, PrecD, isvd_d_val_t, isvd_d_val_t)
