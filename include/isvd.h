@@ -265,11 +265,18 @@ void isvd_dPostprocessGramianNoAlloc(const isvd_Param param, const isvd_d_val_t 
  * ld >= k) -- any of them NULL for zero -- writes dL/dA (da, same storage order as A,
  * ldda >= m for ordera 'C', >= n for 'R').  U, S and V are recomputed from Qbar^T (qt, l x m,
  * the output of the integration stage) by the postprocessing named by alg_p, as in the
- * forward call.  With all l computed triplets:
- *   dA = U [(F o (J - J^T)) S + S (F o (K - K^T)) + diag(gS)] V^T
- *        + (I - U U^T) gU S^-1 V^T + U S^-1 gV^T (I - V V^T),
+ * forward call.  With all l computed triplets and the complement P = I - U U^T:
+ *   dA = U [(F o (J - J^T)) S + S (F o (K - K^T)) + diag(gS)] V^T + E V_k^T + U_k H^T,
  *   J = U^T gU, K = V^T gV, F_ij = 1 / (s_j^2 - s_i^2) for i != j (0 when the gap is below
- *   1e-12 (double) / 1e-6 (single) of max(s_i^2, s_j^2)); gradients padded with zeros to l.
+ *   1e-12 (double) / 1e-6 (single) of max(s_i^2, s_j^2)); gradients padded with zeros to l;
+ *   E = X S_k + P A Y, H = A^T X + Y S_k, with x_i = (s_i^2 I - A_c A_c^T)^-1 P gU_i and
+ *   y_i = (s_i^2 I - A_c^T A_c)^-1 (I - V V^T) gV_i, A_c = P A (summed as Neumann series).
+ * This couples the top-k triplets to every discarded singular pair, so with the exact top-l
+ * subspace it is the derivative of the top-k triplets of the dense SVD of A at any rank.  A
+ * column whose series does not converge (the discarded part of A reaches s_i) keeps the first
+ * term, x_i = P gU_i / s_i^2 (and y_i likewise): the truncated-SVD backward
+ *   (I - U U^T) gU S^-1 V^T + U S^-1 gV^T (I - V V^T)
+ * for that column, which treats the complement as carrying no singular value.
  * ========================================================================================= */
 #define ISVD_DECLARE_BACKWARD(x, T)                                                            \
   void isvd_##x##IsvdBackward(const isvd_Param param, const char *alg_p, const char dista,     \

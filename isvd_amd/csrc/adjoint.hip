@@ -1,19 +1,29 @@
 // Adjoint of the rank-k truncated SVD A ~ U_k S_k V_k^T returned by the driver (no counterpart
 // in the reference).  Given dL/dS (k), dL/dU (m x k) and dL/dV (n x k), it forms dL/dA (m x n)
-// by the SVD backward over all l computed singular triplets (the output gradients padded with
-// zeros from k to l):
+// from the l singular triplets recomputed from Qbar (the output gradients padded with zeros
+// from k to l) and the complement of their span:
 //   J = U^T gU, K = V^T gV (l x l),  F_ij = 1 / (s_j^2 - s_i^2) (i != j, gap-guarded),
 //   Mid = (F o (J - J^T)) S + S (F o (K - K^T)) + diag(gS),
-//   dA = U Mid V^T + (I - U U^T) gU S^-1 V^T + U S^-1 gV^T (I - V V^T).
-// This is the derivative of the top-k singular triplets when A has rank <= l and the l
-// triplets are exact (the complement of span(U) then carries no singular value); otherwise
-// the projection terms stand in for the coupling to the discarded triplets.
+//   dA = U Mid V^T + E V_k^T + U_k H^T,
+//   E = X S_k + P A Y,  H = A^T X + Y S_k,  P = I - U U^T,  A_c = P A,
+//   x_i = (s_i^2 I - A_c A_c^T)^-1 P gU_i,  y_i = (s_i^2 I - A_c^T A_c)^-1 (I - V V^T) gV_i.
+// E and H carry the coupling of the top-k triplets to every discarded singular pair
+// (sum over j > l of u_j (.) / (s_i^2 - s_j^2)), so that with exact top-l triplets this is
+// the derivative of the top-k triplets of the dense SVD at any rank.  x_i and y_i come from
+// the Neumann series s_i^-2 sum_p (A_c A_c^T / s_i^2)^p P gU_i (likewise for y), which
+// converges at the rate (||A_c||_2 / s_i)^2 < 1 when the discarded spectrum lies below s_i;
+// each column is summed until a term falls below the unit roundoff of its sum.  When A_c = 0
+// (rank(A) <= l) the series stops at its first term and E = P gU S_k^-1,
+// H = (I - V V^T) gV S_k^-1: the truncated-SVD backward, which is also what a column keeps
+// when its series does not converge (no gap between s_i and the discarded part of A).
 // U and V are not stored by the forward pass: they are recomputed here from Qbar by the same
 // postprocessing (same kernels, same order of operations), so their signs match the forward
 // outputs the gradients refer to.
 #include "stages.hpp"
 
 #include <cmath>
+#include <cstdlib>
+#include <vector>
 
 #include "la.hpp"
 
@@ -24,6 +34,15 @@ namespace {
 int grid_for(i64 n) {
   i64 b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+}
+
+// ISVD_TRACE=1 prints the complement series' progress to stderr
+bool trace_adjoint() {
+  static const bool on = [] {
+    const char *e = std::getenv("ISVD_TRACE");
+    return e && e[0] == '1';
+  }();
+  return on;
 }
 
 // Mid (l x l) from J, K (l x l, only the first k columns nonzero), s (l) and gS (k)
@@ -50,20 +69,70 @@ __global__ void adjoint_mid_kernel(int l, int k, const T *J, const T *K, const T
   }
 }
 
-// rows r0 .. r0+k-1 of P (ld ldp) := diag(1/s) G (k x cols, ld ldg); zero singular values
-// contribute nothing
+// Rows of a k-row operand selected by a bit mask (k <= 128: bit i of word i / 64)
+struct RowMask {
+  unsigned long long w[2];
+  __host__ __device__ bool has(int i) const { return (w[i >> 6] >> (i & 63)) & 1ull; }
+};
+inline RowMask all_rows(i64 k) {
+  RowMask r{{0ull, 0ull}};
+  for (int i = 0; i < k; ++i) r.w[i >> 6] |= 1ull << (i & 63);
+  return r;
+}
+
+// P (k x cols, ld ldp) := diag(s^e) G (ld ldg) + beta P for e in {-2, -1, 0, 1} on the rows in
+// `rows`; the other rows are left as they are (keep) or zeroed.  A zero singular value
+// contributes nothing to the negative powers.
 template <typename T>
-__global__ void rowscale_inv_kernel(int k, i64 cols, const T *s, const T *G, i64 ldg, T *P,
-                                    i64 ldp, T beta) {
-  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < k * cols;
-       e += (i64)gridDim.x * blockDim.x) {
-    const int i = (int)(e % k);
-    const i64 c = e / k;
-    const T inv = (s[i] != T(0)) ? T(1) / s[i] : T(0);
-    T v = inv * G[i + c * ldg];
+__global__ void rowscale_kernel(int k, i64 cols, const T *s, int e, const T *G, i64 ldg, T *P,
+                                i64 ldp, T beta, RowMask rows, int keep) {
+  for (i64 q = blockIdx.x * (i64)blockDim.x + threadIdx.x; q < k * cols;
+       q += (i64)gridDim.x * blockDim.x) {
+    const int i = (int)(q % k);
+    const i64 c = q / k;
+    if (!rows.has(i)) {
+      if (!keep) P[i + c * ldp] = T(0);
+      continue;
+    }
+    const T si = s[i];
+    T f = T(1);
+    if (e == 1) f = si;
+    else if (e == -1) f = (si != T(0)) ? T(1) / si : T(0);
+    else if (e == -2) f = (si != T(0)) ? T(1) / (si * si) : T(0);
+    T v = f * G[i + c * ldg];
     if (beta != T(0)) v += beta * P[i + c * ldp];
     P[i + c * ldp] = v;
   }
+}
+
+template <typename T>
+void rowscale(i64 k, i64 cols, const T *s, int e, const T *G, i64 ldg, T *P, i64 ldp, T beta,
+              RowMask rows, bool keep = true) {
+  hipLaunchKernelGGL(rowscale_kernel<T>, dim3(grid_for(k * cols)), dim3(256), 0, ctx().stream,
+                     (int)k, cols, s, e, G, ldg, P, ldp, beta, rows, keep ? 1 : 0);
+  LAUNCH_CHECK();
+}
+
+template <typename T>
+void rowscale(i64 k, i64 cols, const T *s, int e, const T *G, i64 ldg, T *P, i64 ldp, T beta) {
+  rowscale<T>(k, cols, s, e, G, ldg, P, ldp, beta, all_rows(k));
+}
+
+// out[i] := sum over columns of P[i, c]^2 for the k rows of P (ld ldp); one workgroup per row,
+// fixed reduction order
+template <typename T>
+__global__ __launch_bounds__(256) void rownorm2_kernel(i64 cols, const T *P, i64 ldp, T *out) {
+  __shared__ T red[4];
+  const int i = blockIdx.x;
+  T part = 0;
+  for (i64 c = threadIdx.x; c < cols; c += 256) {
+    const T v = P[i + c * ldp];
+    part += v * v;
+  }
+  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  __syncthreads();
+  if (threadIdx.x == 0) out[i] = red[0] + red[1] + red[2] + red[3];
 }
 
 }  // namespace
@@ -73,22 +142,38 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
                          const T *qt, i64 ldqt, const T *gs, const T *gut, i64 ldgut,
                          const T *gvt, i64 ldgvt, T *da, i64 ldda) {
   const i64 m = d.m, n = d.n, k = d.k, l = d.l, kl = l + k;
+  // products with A for k-row operands stored transposed (k x m and k x n, ld k):
+  //   xa: W (k x n) := X^T A,  xat: Z (k x m) := W A^T
+  auto xa = [&](const T *x, T *w) {
+    if (ordera == 'C') gemm<T>('N', 'N', k, n, m, T(1), x, k, a, lda, T(0), w, k);
+    else gemm<T>('N', 'T', k, n, m, T(1), x, k, a, lda, T(0), w, k);
+  };
+  auto xat = [&](const T *w, T *z) {
+    if (ordera == 'C') gemm<T>('N', 'T', k, m, n, T(1), w, k, a, lda, T(0), z, k);
+    else gemm<T>('N', 'N', k, m, n, T(1), w, k, a, lda, T(0), z, k);
+  };
   // ---- U, S, V (all l triplets) recomputed from Qbar by the forward's postprocessing
   Dims dl = d;
   dl.k = l;
-  DBuf<T> s(l), pt(kl * m), rt(kl * n);
+  DBuf<T> s(l), pt(kl * m), rt(kl * n), ut_l(l * m), vt_l(l * n);
+  if (tsqr)
+    stage_post_tsqr<T>(dl, ordera, a, lda, qt, ldqt, s.get(), ut_l.get(), l, vt_l.get(), l, true,
+                       true);
+  else
+    stage_post_gramian<T>(dl, ordera, a, lda, qt, ldqt, s.get(), ut_l.get(), l, vt_l.get(), l,
+                          true, true);
+  // z (k x m) := z - (z U) U^T: the component outside span(U)
+  DBuf<T> cproj(k * l);
+  auto project = [&](T *z) {
+    gemm<T>('N', 'T', k, l, m, T(1), z, k, ut_l.get(), l, T(0), cproj.get(), k);
+    gemm<T>('N', 'N', k, m, l, T(-1), cproj.get(), k, ut_l.get(), l, T(1), z, k);
+  };
+  // Pt = [U^T ; E^T]  ((l + k) x m),  Rt = [Mid V^T + [H^T ; 0] ; V_k^T]  ((l + k) x n)
+  omatcopy<T>('N', l, m, T(1), ut_l.get(), l, pt.get(), kl);
+  omatcopy<T>('N', k, n, T(1), vt_l.get(), l, rt.get() + l, kl);
+  DBuf<T> xt(k * m), xp(k * m), zt(k * m), yt(k * n), yp(k * n), wt(k * n);
   {
-    DBuf<T> ut_l(l * m), vt_l(l * n);
-    if (tsqr)
-      stage_post_tsqr<T>(dl, ordera, a, lda, qt, ldqt, s.get(), ut_l.get(), l, vt_l.get(), l,
-                         true, true);
-    else
-      stage_post_gramian<T>(dl, ordera, a, lda, qt, ldqt, s.get(), ut_l.get(), l, vt_l.get(), l,
-                            true, true);
-    // Pt = [U^T ; S_k^-1 Gu^T]  ((l + k) x m),  Rt = [R1 ; V_k^T]  ((l + k) x n)
-    omatcopy<T>('N', l, m, T(1), ut_l.get(), l, pt.get(), kl);
-    omatcopy<T>('N', k, n, T(1), vt_l.get(), l, rt.get() + l, kl);
-    DBuf<T> j(l * l), kk(l * l), mid(l * l), gu(k * m), gv(k * n);
+    DBuf<T> j(l * l), kk(l * l), mid(l * l);
     // J[:, :k] = U^T gU, K[:, :k] = V^T gV (the remaining columns are zero)
     fill<T>(l, l, T(0), j.get(), l);
     fill<T>(l, l, T(0), kk.get(), l);
@@ -99,22 +184,87 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
                        (int)l, (int)k, (const T *)j.get(), (const T *)kk.get(),
                        (const T *)s.get(), gs, delta, mid.get());
     LAUNCH_CHECK();
-    // Gu^T = gU^T - J[:, :k]^T U^T  (k x m);  Gv^T = gV^T - K[:, :k]^T V^T  (k x n)
-    omatcopy<T>('N', k, m, T(1), gut, ldgut, gu.get(), k);
-    gemm<T>('T', 'N', k, m, l, T(-1), j.get(), l, ut_l.get(), l, T(1), gu.get(), k);
-    omatcopy<T>('N', k, n, T(1), gvt, ldgvt, gv.get(), k);
-    gemm<T>('T', 'N', k, n, l, T(-1), kk.get(), l, vt_l.get(), l, T(1), gv.get(), k);
-    // Pt rows l..l+k-1 := S_k^-1 Gu^T
-    hipLaunchKernelGGL(rowscale_inv_kernel<T>, dim3(grid_for(k * m)), dim3(256), 0, ctx().stream,
-                       (int)k, m, (const T *)s.get(), (const T *)gu.get(), k, pt.get() + l, kl,
-                       T(0));
-    LAUNCH_CHECK();
-    // R1 = Mid V^T + [S_k^-1 Gv^T ; 0]  (l x n) into Rt rows 0..l-1
+    // P gU (as its transpose, into zt) and (I - V V^T) gV (into wt)
+    omatcopy<T>('N', k, m, T(1), gut, ldgut, zt.get(), k);
+    gemm<T>('T', 'N', k, m, l, T(-1), j.get(), l, ut_l.get(), l, T(1), zt.get(), k);
+    omatcopy<T>('N', k, n, T(1), gvt, ldgvt, wt.get(), k);
+    gemm<T>('T', 'N', k, n, l, T(-1), kk.get(), l, vt_l.get(), l, T(1), wt.get(), k);
+    // R1 = Mid V^T into Rt rows 0..l-1
     gemm<T>('N', 'N', l, n, l, T(1), mid.get(), l, vt_l.get(), l, T(0), rt.get(), kl);
-    hipLaunchKernelGGL(rowscale_inv_kernel<T>, dim3(grid_for(k * n)), dim3(256), 0, ctx().stream,
-                       (int)k, n, (const T *)s.get(), (const T *)gv.get(), k, rt.get(), kl, T(1));
-    LAUNCH_CHECK();
   }
+  // ---- the complement: X = sum_p X_p, X_0 = P gU S^-2, X_{p+1} = P A A^T X_p S^-2, and
+  // Y = sum_p Y_p, Y_0 = (I - V V^T) gV S^-2, Y_{p+1} = A^T P A Y_p S^-2 (transposed storage).
+  // Column i of either series converges at the rate (||A_c||_2 / s_i)^2.  Each column is summed
+  // until its term falls below the unit roundoff of its sum (or for kMaxTerms terms); a column
+  // whose terms stop shrinking has no gap below s_i (the discarded part of A reaches s_i, e.g.
+  // a subspace from a sketch too small for the spectrum) and keeps its first term only: the
+  // truncated-SVD backward, which treats the complement as carrying no singular value.
+  DBuf<T> x0(k * m), y0(k * n);
+  rowscale<T>(k, m, s.get(), -2, zt.get(), k, x0.get(), k, T(0));
+  rowscale<T>(k, m, s.get(), 0, x0.get(), k, xp.get(), k, T(0));
+  rowscale<T>(k, m, s.get(), 0, x0.get(), k, xt.get(), k, T(0));
+  rowscale<T>(k, n, s.get(), -2, wt.get(), k, y0.get(), k, T(0));
+  rowscale<T>(k, n, s.get(), 0, y0.get(), k, yp.get(), k, T(0));
+  rowscale<T>(k, n, s.get(), 0, y0.get(), k, yt.get(), k, T(0));
+  DBuf<T> nrm(4 * k);
+  std::vector<T> h(4 * k);
+  const double eps = (sizeof(T) == 8) ? 0x1p-52 : 0x1p-23;
+  constexpr int kMaxTerms = 300;
+  RowMask ax = all_rows(k), ay = all_rows(k);
+  std::vector<double> prev_x(k, INFINITY), prev_y(k, INFINITY);
+  auto any = [](const RowMask &r) { return (r.w[0] | r.w[1]) != 0ull; };
+  for (int it = 1; any(ax) || any(ay); ++it) {
+    xa(xp.get(), wt.get());
+    xat(wt.get(), zt.get());
+    project(zt.get());
+    rowscale<T>(k, m, s.get(), -2, zt.get(), k, xp.get(), k, T(0), ax, false);
+    rowscale<T>(k, m, s.get(), 0, xp.get(), k, xt.get(), k, T(1), ax);
+    xat(yp.get(), zt.get());
+    project(zt.get());
+    xa(zt.get(), wt.get());
+    rowscale<T>(k, n, s.get(), -2, wt.get(), k, yp.get(), k, T(0), ay, false);
+    rowscale<T>(k, n, s.get(), 0, yp.get(), k, yt.get(), k, T(1), ay);
+    hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, m,
+                       (const T *)xp.get(), k, nrm.get());
+    hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, m,
+                       (const T *)xt.get(), k, nrm.get() + k);
+    hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, n,
+                       (const T *)yp.get(), k, nrm.get() + 2 * k);
+    hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, n,
+                       (const T *)yt.get(), k, nrm.get() + 3 * k);
+    LAUNCH_CHECK();
+    read_back(h.data(), nrm.get(), sizeof(T) * 4 * k);
+    RowMask rx{{0ull, 0ull}}, ry{{0ull, 0ull}};  // columns reverting to their first term
+    auto judge = [&](RowMask &act, RowMask &rev, std::vector<double> &prev, const T *term,
+                     const T *sum) {
+      for (int i = 0; i < (int)k; ++i) {
+        if (!act.has(i)) continue;
+        const double t = std::sqrt((double)term[i]), sm = std::sqrt((double)sum[i]);
+        const bool done = t <= eps * sm || it >= kMaxTerms;
+        const bool grows = !(t < prev[i]);
+        prev[i] = t;
+        if (done || grows) act.w[i >> 6] &= ~(1ull << (i & 63));
+        if (grows && !done) rev.w[i >> 6] |= 1ull << (i & 63);
+      }
+    };
+    judge(ax, rx, prev_x, h.data(), h.data() + k);
+    judge(ay, ry, prev_y, h.data() + 2 * k, h.data() + 3 * k);
+    if (any(rx)) rowscale<T>(k, m, s.get(), 0, x0.get(), k, xt.get(), k, T(0), rx);
+    if (any(ry)) rowscale<T>(k, n, s.get(), 0, y0.get(), k, yt.get(), k, T(0), ry);
+    if (trace_adjoint())
+      std::fprintf(stderr, "isvd adjoint complement term %d: %d / %d columns still summed\n", it,
+                   __builtin_popcountll(ax.w[0]) + __builtin_popcountll(ax.w[1]) +
+                       __builtin_popcountll(ay.w[0]) + __builtin_popcountll(ay.w[1]),
+                   (int)(2 * k));
+  }
+  // E^T = S X^T + (P A Y)^T into Pt rows l..l+k-1;  H^T = X^T A + S Y^T added to Rt rows 0..k-1
+  xat(yt.get(), zt.get());
+  project(zt.get());
+  rowscale<T>(k, m, s.get(), 0, zt.get(), k, pt.get() + l, kl, T(0));
+  rowscale<T>(k, m, s.get(), 1, xt.get(), k, pt.get() + l, kl, T(1));
+  xa(xt.get(), wt.get());
+  rowscale<T>(k, n, s.get(), 1, yt.get(), k, wt.get(), k, T(1));
+  rowscale<T>(k, n, s.get(), 0, wt.get(), k, rt.get(), kl, T(1));
   // dA = Pt^T Rt  (m x n; for a row-major A the transpose n x m is formed instead)
   if (ordera == 'C')
     gemm<T>('T', 'N', m, n, kl, T(1), pt.get(), kl, rt.get(), kl, T(0), da, ldda);

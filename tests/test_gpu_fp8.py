@@ -165,9 +165,10 @@ def test_driver_fp8_sketch_vs_fp64_driver(torch_cuda, algs):
     print(f"fp8 vs fp64 driver {algs}: sigma rel {s_err:.3e}, angle U {ang_u:.3e}, "
           f"V {ang_v:.3e}; fp64 driver vs dense SVD: sigma rel "
           f"{np.max(np.abs(sd - ss) / ss):.3e}; iter e {re['iter']} d {rd['iter']}")
-    # measured on gfx950: sigma 2.6e-4, angles 3.1e-2 (U) and 7.3e-3 (V); the FP64 driver
-    # itself is 1.3e-2 away from the dense SVD's sigma at this spectrum and sketch size
-    assert s_err < 1e-3 and ang_u < 0.1 and ang_v < 0.03
+    # measured on gfx950: sigma 2.56e-4, angles 3.11e-2 (U) and 7.34e-3 (V), bounded here with
+    # a 45 % margin; the FP64 driver itself is 1.3e-2 away from the dense SVD's sigma at this
+    # spectrum and sketch size
+    assert s_err < 3.7e-4 and ang_u < 0.045 and ang_v < 0.0107
 
 
 def test_host_arrays_are_staged_like_device_arrays(torch_cuda):

@@ -102,3 +102,81 @@ def test_seeded_omega_is_standard_gaussian(torch_cuda):
     y = host(pipeline.sketch(prm, eye, seed=7))  # Y = I * Omega = Omega (n x Nl)
     assert abs(y.mean()) < 0.01 and abs(y.std() - 1) < 0.01
     assert abs(np.mean(y ** 4) - 3) < 0.1
+
+
+# ---- the 131072 x 16384 workload's sketch count and size (N = 32, l = 128, Nl = 4096) on a
+# reduced A, so that the oracle (full-space KN / WY iterations over m x Nl bases) finishes in
+# about 10-20 s per integration on the CPU
+N32 = dict(m=8192, n=4096, k=100, p=28, N=32)
+
+
+def n32_problem():
+    m, n, k, p, N = (N32[x] for x in ("m", "n", "k", "p", "N"))
+    a = lowrank_matrix(m, n, 1.0 / np.arange(1, 2049), seed=12, noise=1e-6)
+    om = np.random.default_rng(13).standard_normal((n, N * (k + p)))
+    return a, om
+
+
+@pytest.mark.parametrize("ordera", ["C", "R"])
+def test_n32_kolmogorov_nagumo_matches_oracle(torch_cuda, ordera):
+    """GP / TS / KN / TS at N = 32, l = 128: the FP64 bar (sigma 1e-10, angles 1e-8, same
+    iteration count).  Exercises the Nl x Nl = 4096 x 4096 Gram (upper tiles, split-K), the
+    transposed-store M B product and the 32-sketch TSQR tree."""
+    a, om = n32_problem()
+    g, r = run_both(torch_cuda, a, om, N32["k"], N32["p"], N32["N"], ("GP", "TS", "KN", "TS"),
+                    ordera=ordera)
+    print(f"N=32 KN ordera={ordera}: iter {g['iter']} (oracle {r['iter']})")
+    check(g, r, N32["k"])
+
+
+def test_n32_wen_yin_within_the_oracles_own_rounding_sensitivity(torch_cuda):
+    """GP / TS / WY / TS at N = 32, l = 128.  Wen-Yin's Barzilai-Borwein steps and nonmonotone
+    line search amplify rounding, so the tolerance is measured here: the oracle is run twice,
+    on Qs and on Qs perturbed by one unit roundoff (2^-52 relative, elementwise), and the GPU
+    result (whose products round differently everywhere) must lie within 10x that spread of
+    the oracle (never looser than sigma 1e-8 / angle 1e-6, never tighter than the FP64 bar),
+    with the iteration count within the spread plus one."""
+    k, N = N32["k"], N32["N"]
+    a, om = n32_problem()
+    g, r = run_both(torch_cuda, a, om, k, N32["p"], N, ("GP", "TS", "WY", "TS"))
+    rng = np.random.default_rng(14)
+    qs2 = r["qs"] * (1.0 + 2.0 ** -52 * rng.standard_normal(r["qs"].shape))
+    q2, it2, _ = O.integrate_wen_yin(qs2, N)
+    s2, u2, v2 = O.postprocess_tall_skinny_qr(a, q2, k)
+    ds = np.abs(s2[:k] - r["s"][:k]).max() / r["s"][0]
+    da = max(principal_angles(u2, r["u"]).max(), principal_angles(v2, r["v"]).max())
+    s_tol = min(max(10 * ds, 1e-10), 1e-8)
+    a_tol = min(max(10 * da, 1e-8), 1e-6)
+    slack = abs(it2 - r["iter"]) + 1
+    s = host(g["s"])
+    gs = np.abs(s[:k] - r["s"][:k]).max() / r["s"][0]
+    ga = max(principal_angles(host(g["u"])[:, :k], r["u"]).max(),
+             principal_angles(host(g["v"])[:, :k], r["v"]).max())
+    print(f"N=32 WY: oracle self-spread sigma {ds:.2e} angle {da:.2e} iter {it2} vs "
+          f"{r['iter']}; GPU sigma {gs:.2e} angle {ga:.2e} iter {g['iter']}")
+    assert abs(g["iter"] - r["iter"]) <= slack
+    assert gs <= s_tol and ga <= a_tol, (gs, s_tol, ga, a_tol)
+
+
+def test_n32_mixed_precision_matches_fp64_oracle(torch_cuda):
+    """The b family (BF16 A and Omega; GR twice in FP32, split-BF16 Gram, FP64 l x l solves,
+    three-term BF16 projection) at N = 32, l = 128 against the FP64 oracle on the identical
+    BF16 values: sigma within 1e-5 relative and principal angles within 1e-4 (the mixed path's
+    accuracy target)."""
+    torch = torch_cuda
+    k, p, N = N32["k"], N32["p"], N32["N"]
+    a, om = n32_problem()
+    a_b = torch.from_numpy(a.astype(np.float32)).cuda().to(torch.bfloat16)
+    om_b = torch.from_numpy(om.astype(np.float32)).cuda().to(torch.bfloat16)
+    a64, om64 = host(a_b), host(om_b)
+    prm = capi.create_param(a.shape[0], a.shape[1], k, p, N)
+    algs = ("GP", "GR", "KN", "TS")
+    g = pipeline.isvd(prm, a_b, omega=om_b, algs=algs)
+    r = O.isvd(a64, om64, k, N, "GR", "KN", "TS")
+    s = host(g["s"])
+    gs = np.abs(s[:k] - r["s"][:k]).max() / r["s"][0]
+    ga = max(principal_angles(host(g["u"])[:, :k], r["u"]).max(),
+             principal_angles(host(g["v"])[:, :k], r["v"]).max())
+    print(f"N=32 mixed vs FP64 oracle: sigma {gs:.2e} angle {ga:.2e} iter {g['iter']} "
+          f"(oracle {r['iter']})")
+    assert gs <= 1e-5 and ga <= 1e-4, (gs, ga)
