@@ -155,9 +155,10 @@ def exact_factors(torch, m, n, seed):
 
     def orth(rows, cols):
         x = torch.randn(rows, cols, generator=g, device="cuda", dtype=torch.float64)
-        for _ in range(2):  # x := x R^-1 with R^T R = x^T x (R^T = c, lower)
+        eye = torch.eye(cols, device="cuda", dtype=torch.float64)
+        for _ in range(2):  # x := x R^-1 with R^T R = x^T x (R = c^T)
             c = torch.linalg.cholesky(x.T @ x)
-            x = torch.linalg.solve_triangular(c, x.T, upper=False).T.contiguous()
+            x = x @ torch.linalg.solve_triangular(c.T, eye, upper=True)
         return x
 
     return orth(m, n), orth(n, n)
