@@ -78,7 +78,16 @@ isvd_Param isvd_createParam(const isvd_int_t nrow, const isvd_int_t ncol, const 
                             const mpi_int_t mpi_root, const isvd_MpiComm mpi_comm);
 
 /* ---- environment: src/include/c/isvd/core/env.h:30-55, src/include/c/isvd/gpu/env.h:19-25 ---- */
-extern size_t isvd_gpu_memory_limit; /* gpu/env.h:19; bytes of HBM the library may use (0 = all) */
+/* gpu/env.h:19.  Bytes of HBM for staging a host-resident A (0 = no limit).  As in the
+ * reference's GPU stages (gpu/stage/@x@_sketch_gpu.h:63-77, @x@_postprocess_gpu.h:73-82), the
+ * sketch and the projection Z^T = Qbar^T A of the s and d families then stage A by blocks of
+ * columns (multiples of 64) that fit min(free HBM, limit) beside the buffers resident through
+ * the stage (the sketch, the bases, Omega^T, Z^T, the outputs), and fail with an error when not
+ * even one column fits.  A device-resident A is used in place and not counted; the b and e
+ * families and symmetric postprocessing stage a host-resident A whole and fail when it does not
+ * fit.  The integration stages' own workspace is not counted (the reference runs them on the
+ * CPU). */
+extern size_t isvd_gpu_memory_limit;
 void isvd_Init_cpu(int *argcp, char ***argvp, const isvd_MpiComm mpi_comm);
 void isvd_Finalize_cpu(void);
 void isvd_PrintEnvironment_cpu(const isvd_MpiComm comm);

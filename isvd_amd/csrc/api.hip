@@ -187,6 +187,26 @@ inline i64 span(i64 m, i64 n, i64 ld) { return (m <= 0 || n <= 0) ? 0 : (n - 1) 
 // e4m3 rows are read and written in whole 16-byte chunks: a row of n codes spans roundup(n, 16)
 inline i64 round16(i64 n) { return (n + 15) / 16 * 16; }
 
+// Columns of a host-resident A staged per block under isvd_gpu_memory_limit, as in the
+// reference's GPU stages (src/lib/libisvd/gpu/stage/@x@_sketch_gpu.h:63-77,
+// @x@_postprocess_gpu.h:73-82): the budget min(free HBM, limit), less the bytes of the device
+// buffers that stay resident through the stage, over the bytes of one column of A, in
+// multiples of 64 columns.  n (A staged whole) when no limit is set or all of A fits.
+static i64 host_block_cols(i64 n, i64 col_bytes, i64 resident_bytes) {
+  if (isvd_gpu_memory_limit == 0) return n;
+  size_t free_b = 0, total_b = 0;
+  HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const size_t budget = free_b < isvd_gpu_memory_limit ? free_b : isvd_gpu_memory_limit;
+  ISVD_CHECK(budget >= (size_t)(resident_bytes + col_bytes),
+             "not enough GPU memory: at least %lld bytes are needed (%lld resident + one column "
+             "of A), %zu are available under isvd_gpu_memory_limit",
+             resident_bytes + col_bytes, resident_bytes, budget);
+  i64 cols = (i64)((budget - (size_t)resident_bytes) / (size_t)col_bytes);
+  if (cols >= n) return n;
+  if (cols > 64) cols = cols / 64 * 64;
+  return cols;
+}
+
 // ---------------------------------------------------------------------------------------------
 // argument parsing: isvd_arg2char / isvd_arg2str (src/lib/libisvd/util/arg.h:22-79)
 // ---------------------------------------------------------------------------------------------
@@ -291,7 +311,15 @@ static void sketch_impl(const isvd_Param &param, char dista, char ordera,
     ISVD_CHECK(ldyst >= d.Nl, "ldyst (%lld) must be >= Nl (%lld)", ldyst, d.Nl);
   ensure_init();
   const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
-  Arg<TA> A(a, a_span, true, false);
+  // a host-resident A beyond the memory limit is staged by column blocks
+  const bool a_host = a != nullptr && !is_device_pointer(a);
+  const i64 cols = a_host ? host_block_cols(d.n, d.m * (i64)sizeof(TA),
+                                            d.Nl * (d.m * (i64)sizeof(T) + d.n * (i64)sizeof(TA)))
+                          : d.n;
+  ISVD_CHECK(cols == d.n || !P::mixed,
+             "the mixed-precision family stages a host-resident A whole: raise "
+             "isvd_gpu_memory_limit or pass device memory");
+  Arg<TA> A(cols == d.n ? a : nullptr, a_span, true, false);
   Arg<T> Y(yst, span(d.Nl, d.m, ldyst), false, true);
   DBuf<TA> om;
   const TA *omd;
@@ -307,6 +335,7 @@ static void sketch_impl(const isvd_Param &param, char dista, char ordera,
     omd = O.dev;
   }
   if constexpr (P::mixed) stage_sketch_b(d, ordera_, A.dev, lda, omd, ldo, Y.dev, ldyst);
+  else if (cols < d.n) stage_sketch_host_blocks<T>(d, ordera_, a, lda, omd, ldo, Y.dev, ldyst, cols);
   else stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, Y.dev, ldyst);
   Y.finish();
   sync();
@@ -465,7 +494,18 @@ static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera
   }
   ensure_init();
   const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
-  Arg<TA> A(a, a_span, true, false);
+  // a host-resident A beyond the memory limit is staged by column blocks; Qbar^T, Z^T and the
+  // outputs stay resident
+  const bool a_host = a != nullptr && !is_device_pointer(a);
+  const i64 sz = (i64)sizeof(T);
+  const i64 cols =
+      a_host ? host_block_cols(d.n, d.m * (i64)sizeof(TA),
+                               sz * (d.l * d.m + d.l * d.n + d.k * (d.m + d.n) + 4 * d.l * d.l))
+             : d.n;
+  ISVD_CHECK(cols == d.n || (!P::mixed && kind != 'S'),
+             "this postprocessing stages a host-resident A whole: raise isvd_gpu_memory_limit "
+             "or pass device memory");
+  Arg<TA> A(cols == d.n ? a : nullptr, a_span, true, false);
   Arg<T> Q(qt, span(d.l, d.m, ldqt), true, false);
   Arg<T> S(s, d.l, false, true);
   if constexpr (!P::mixed) {
@@ -486,15 +526,24 @@ static void post_impl(const isvd_Param &param, i64 argc, char dista, char ordera
   }
   Arg<T> U(want_u ? ut : nullptr, span(d.k, d.m, ldut), false, true);
   Arg<T> V(want_v ? vt : nullptr, span(d.k, d.n, ldvt), false, true);
-  if constexpr (P::mixed)
+  if constexpr (P::mixed) {
     stage_post_b(d, tsqr, ordera_, A.dev, lda, Q.dev, ldqt, S.dev, U.dev, ldut, V.dev, ldvt,
                  want_u, want_v);
-  else if (tsqr)
+  } else if (cols < d.n) {
+    DBuf<T> zt(d.l * d.n);
+    project_host_blocks<T>(d, ordera_, a, lda, Q.dev, ldqt, zt.get(), cols);
+    if (tsqr)
+      post_tsqr_z<T>(d, Q.dev, ldqt, zt.get(), S.dev, U.dev, ldut, V.dev, ldvt, want_u, want_v);
+    else
+      post_gramian_z<T>(d, Q.dev, ldqt, zt.get(), S.dev, U.dev, ldut, V.dev, ldvt, want_u,
+                        want_v);
+  } else if (tsqr) {
     stage_post_tsqr<T>(d, ordera_, A.dev, lda, Q.dev, ldqt, S.dev, U.dev, ldut, V.dev, ldvt,
                        want_u, want_v);
-  else
+  } else {
     stage_post_gramian<T>(d, ordera_, A.dev, lda, Q.dev, ldqt, S.dev, U.dev, ldut, V.dev, ldvt,
                           want_u, want_v);
+  }
   S.finish();
   U.finish();
   V.finish();
@@ -669,7 +718,20 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   (void)argc_s; (void)argc_o; (void)argc_p;
   ensure_init();
   const i64 a_span = ordera_ == 'C' ? span(d.m, d.n, lda) : span(d.n, d.m, lda);
-  Arg<TA> A(a, a_span, true, false);
+  // a host-resident A beyond the memory limit is staged by column blocks, once for the sketch
+  // and once for the projection; the sketch, the bases, Omega^T, Z^T and the outputs stay
+  // resident
+  const bool a_host = a != nullptr && !is_device_pointer(a) && !e8;
+  const i64 sz = (i64)sizeof(T);
+  const i64 cols = a_host ? host_block_cols(d.n, d.m * (i64)sizeof(TA),
+                                            sz * (d.Nl * d.m + d.l * d.m + d.l * d.n +
+                                                  d.k * (d.m + d.n)) +
+                                                (i64)sizeof(TA) * d.Nl * d.n)
+                          : d.n;
+  ISVD_CHECK(cols == d.n || (!P::mixed && !sy),
+             "this decomposition stages a host-resident A whole: raise isvd_gpu_memory_limit "
+             "or pass device memory");
+  Arg<TA> A(cols == d.n ? a : nullptr, a_span, true, false);
   Arg<T> S(s, d.l, false, true);
   Arg<T> U(want_u ? ut : nullptr, span(d.k, d.m, ldut), false, true);
   Arg<T> V(want_v ? vt : nullptr, span(d.k, d.n, ldvt), false, true);
@@ -712,6 +774,8 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
       omd = O.dev;
     }
     if constexpr (P::mixed) stage_sketch_b(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
+    else if (cols < d.n)
+      stage_sketch_host_blocks<T>(d, ordera_, a, lda, omd, ldo, yst.get(), d.Nl, cols);
     else stage_sketch<T>(d, ordera_, A.dev, lda, omd, ldo, yst.get(), d.Nl);
   }
   HIP_CHECK(hipEventRecord(ev[1], ctx().stream));
@@ -743,12 +807,21 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
     stage_post_symmetric<T>(d, ordera_, A.dev, lda, qt.get(), d.l, S.dev,
                             want_u ? U.dev : V.dev, want_u ? ldut : ldvt, want);
     if (want_u && want_v) omatcopy<T>('N', d.k, d.m, T(1), U.dev, ldut, V.dev, ldvt);
-  } else if (ap[0] == 'T')
+  } else if (cols < d.n) {
+    DBuf<T> zt(d.l * d.n);
+    project_host_blocks<T>(d, ordera_, a, lda, qt.get(), d.l, zt.get(), cols);
+    if (ap[0] == 'T')
+      post_tsqr_z<T>(d, qt.get(), d.l, zt.get(), S.dev, U.dev, ldut, V.dev, ldvt, want_u, want_v);
+    else
+      post_gramian_z<T>(d, qt.get(), d.l, zt.get(), S.dev, U.dev, ldut, V.dev, ldvt, want_u,
+                        want_v);
+  } else if (ap[0] == 'T') {
     stage_post_tsqr<T>(d, ordera_, A.dev, lda, qt.get(), d.l, S.dev, U.dev, ldut, V.dev, ldvt,
                        want_u, want_v);
-  else
+  } else {
     stage_post_gramian<T>(d, ordera_, A.dev, lda, qt.get(), d.l, S.dev, U.dev, ldut, V.dev, ldvt,
                           want_u, want_v);
+  }
   HIP_CHECK(hipEventRecord(ev[4], ctx().stream));
   S.finish();
   U.finish();

@@ -214,6 +214,50 @@ void stage_sketch(const Dims &d, char ordera, const T *a, i64 lda, const T *omeg
   gemm<T>('N', transa, d.Nl, d.m, d.n, T(1), omegat, ldo, a, lda, T(0), yst, ldyst);
 }
 
+// A host-resident A staged by column blocks of `cols` columns (the reference's GPU stages,
+// src/lib/libisvd/gpu/stage/@x@_sketch_gpu.h:104-125, @x@_postprocess_gpu.h:121-140).  Block
+// c0 .. c0 + nc of A is an m x nc column-major block for ordera 'C', or the nc x m block of the
+// row-major array (rows c0 .. c0 + nc of A^T) for 'R'; it lands in `buf` with leading dimension
+// m ('C') or nc ('R').
+template <typename T>
+static i64 stage_block(char ordera, i64 m, const T *a_host, i64 lda, i64 c0, i64 nc, T *buf) {
+  if (ordera == 'C') {
+    HIP_CHECK(hipMemcpy2DAsync(buf, sizeof(T) * m, a_host + c0 * lda, sizeof(T) * lda,
+                               sizeof(T) * m, nc, hipMemcpyHostToDevice, ctx().stream));
+    return m;
+  }
+  HIP_CHECK(hipMemcpy2DAsync(buf, sizeof(T) * nc, a_host + c0, sizeof(T) * lda, sizeof(T) * nc,
+                             m, hipMemcpyHostToDevice, ctx().stream));
+  return nc;
+}
+
+// Y^T := Omega^T op(A)^T accumulated over column blocks of a host-resident A
+// (@x@_sketch_gpu.h:116-125: Y^T += Omega_b^T A_b^T)
+template <typename T>
+void stage_sketch_host_blocks(const Dims &d, char ordera, const T *a_host, i64 lda,
+                              const T *omegat, i64 ldo, T *yst, i64 ldyst, i64 cols) {
+  DBuf<T> buf(d.m * cols);
+  for (i64 c0 = 0; c0 < d.n; c0 += cols) {
+    const i64 nc = (d.n - c0) < cols ? (d.n - c0) : cols;
+    const i64 ldb = stage_block<T>(ordera, d.m, a_host, lda, c0, nc, buf.get());
+    gemm<T>('N', ordera == 'C' ? 'T' : 'N', d.Nl, d.m, nc, T(1), omegat + c0 * ldo, ldo,
+            buf.get(), ldb, c0 ? T(1) : T(0), yst, ldyst);
+  }
+}
+
+// Z^T := Qbar^T op(A) by column blocks of a host-resident A (@x@_postprocess_gpu.h:121-140)
+template <typename T>
+void project_host_blocks(const Dims &d, char ordera, const T *a_host, i64 lda, const T *qt,
+                         i64 ldqt, T *zt, i64 cols) {
+  DBuf<T> buf(d.m * cols);
+  for (i64 c0 = 0; c0 < d.n; c0 += cols) {
+    const i64 nc = (d.n - c0) < cols ? (d.n - c0) : cols;
+    const i64 ldb = stage_block<T>(ordera, d.m, a_host, lda, c0, nc, buf.get());
+    gemm<T>('N', ordera == 'C' ? 'N' : 'T', d.l, nc, d.m, T(1), qt, ldqt, buf.get(), ldb, T(0),
+            zt + c0 * d.l, d.l);
+  }
+}
+
 // Gramian orthogonalization (src/lib/libisvd/core/stage/@x@_orthogonalize_gramian.c:81-98).
 template <typename T>
 void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst) {
@@ -657,6 +701,10 @@ void stage_post_symmetric(const Dims &d, char ordera, const T *a, i64 lda, const
                                         T *, T *, i64, bool);                                  \
   template void gaussian_omega<T>(i64, i64, T *, i64, unsigned long long);                    \
   template void stage_sketch<T>(const Dims &, char, const T *, i64, const T *, i64, T *, i64); \
+  template void stage_sketch_host_blocks<T>(const Dims &, char, const T *, i64, const T *, i64,  \
+                                            T *, i64, i64);                                    \
+  template void project_host_blocks<T>(const Dims &, char, const T *, i64, const T *, i64, T *,  \
+                                       i64);                                                   \
   template void stage_orth_gramian<T>(const Dims &, T *, i64);                                \
   template void stage_orth_tsqr<T>(const Dims &, T *, i64);                                   \
   template void stage_integrate_kn<T>(const Dims &, const KnArgs &, const T *, i64, T *, i64,  \

@@ -31,6 +31,15 @@ template <typename T>
 void stage_sketch(const Dims &d, char ordera, const T *a, i64 lda, const T *omegat, i64 ldo,
                   T *yst, i64 ldyst);
 
+// The sketch and the projection Z^T = Qbar^T op(A) for a host-resident A staged through HBM
+// by blocks of `cols` columns (the memory-limited path of the reference's GPU stages)
+template <typename T>
+void stage_sketch_host_blocks(const Dims &d, char ordera, const T *a_host, i64 lda,
+                              const T *omegat, i64 ldo, T *yst, i64 ldyst, i64 cols);
+template <typename T>
+void project_host_blocks(const Dims &d, char ordera, const T *a_host, i64 lda, const T *qt,
+                         i64 ldqt, T *zt, i64 cols);
+
 template <typename T>
 void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst);
 

@@ -201,5 +201,6 @@ def test_differentiation_workload_full_size(torch_cuda):
     loss = (w.float() * s_).sum() + (gU.float() * u_).sum() + (gV.float() * v_).sum()
     loss.backward()
     assert ad.grad.shape == (m, n) and bool(torch.isfinite(ad.grad).all())
+    u_, v_ = u_.detach(), v_.detach()
     assert float((u_.T @ u_ - torch.eye(k, device="cuda")).abs().max()) < 1e-4
     assert float((v_.T @ v_ - torch.eye(k, device="cuda")).abs().max()) < 1e-4
