@@ -69,6 +69,9 @@ void ensure_init() {
   // A blocking stream: work the caller queued on the legacy default stream (e.g. the kernels
   // that produced device-resident inputs) completes before the library's kernels start.
   HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamDefault));
+  HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.aux, hipStreamNonBlocking));
+  HIP_CHECK(hipEventCreateWithFlags(&g_ctx.fork, hipEventDisableTiming));
+  HIP_CHECK(hipEventCreateWithFlags(&g_ctx.join, hipEventDisableTiming));
   // Stream-ordered buffers come from the device's default pool.  Keep freed memory cached in
   // the pool across synchronizations (the integration loops synchronize once per iteration):
   // with the default threshold of 0 every synchronization returns it to the driver and the
@@ -128,20 +131,70 @@ static void *g_pinned = nullptr;
 static hipEvent_t g_readback_ev = nullptr;
 constexpr size_t kPinnedBytes = 4096;
 
-void read_back(void *host, const void *dev, size_t bytes) {
-  ISVD_CHECK(bytes <= kPinnedBytes, "read_back: %zu bytes is too many", bytes);
+// pinned bytes: read_back from the start, then kSlots slots of kSlotBytes at the end
+constexpr size_t kSlotBytes = 256;
+constexpr int kSlots = 4;
+static hipEvent_t g_slot_ev[kSlots] = {};
+
+static void ensure_pinned() {
   if (!g_pinned) {
     HIP_CHECK(hipHostMalloc(&g_pinned, kPinnedBytes, hipHostMallocDefault));
     HIP_CHECK(hipEventCreateWithFlags(&g_readback_ev, hipEventDisableTiming));
+    for (int i = 0; i < kSlots; ++i)
+      HIP_CHECK(hipEventCreateWithFlags(&g_slot_ev[i], hipEventDisableTiming));
   }
-  HIP_CHECK(hipMemcpyAsync(g_pinned, dev, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
-  HIP_CHECK(hipEventRecord(g_readback_ev, g_ctx.stream));
+}
+
+static void release_pinned() {
+  if (!g_pinned) return;
+  HIP_CHECK(hipEventDestroy(g_readback_ev));
+  for (int i = 0; i < kSlots; ++i) HIP_CHECK(hipEventDestroy(g_slot_ev[i]));
+  HIP_CHECK(hipHostFree(g_pinned));
+  g_pinned = nullptr;
+  g_readback_ev = nullptr;
+  for (int i = 0; i < kSlots; ++i) g_slot_ev[i] = nullptr;
+}
+
+static void spin_on(hipEvent_t ev) {
   for (;;) {
-    const hipError_t e = hipEventQuery(g_readback_ev);
+    const hipError_t e = hipEventQuery(ev);
     if (e == hipSuccess) break;
     if (e != hipErrorNotReady) HIP_CHECK(e);
   }
+}
+
+void read_back(void *host, const void *dev, size_t bytes) {
+  ISVD_CHECK(bytes <= kPinnedBytes - kSlots * kSlotBytes, "read_back: %zu bytes is too many",
+             bytes);
+  ensure_pinned();
+  HIP_CHECK(hipMemcpyAsync(g_pinned, dev, bytes, hipMemcpyDeviceToHost, g_ctx.stream));
+  HIP_CHECK(hipEventRecord(g_readback_ev, g_ctx.stream));
+  spin_on(g_readback_ev);
   std::memcpy(host, g_pinned, bytes);
+}
+
+static unsigned char *slot_ptr(int slot) {
+  return static_cast<unsigned char *>(g_pinned) + kPinnedBytes - (size_t)(kSlots - slot) * kSlotBytes;
+}
+
+void *pinned_slot(int slot) {
+  ISVD_CHECK(slot >= 0 && slot < kSlots, "pinned_slot: bad slot");
+  ensure_pinned();
+  void *dev = nullptr;
+  HIP_CHECK(hipHostGetDevicePointer(&dev, slot_ptr(slot), 0));
+  return dev;
+}
+
+void pinned_slot_record(int slot) {
+  ISVD_CHECK(slot >= 0 && slot < kSlots, "pinned_slot_record: bad slot");
+  ensure_pinned();
+  HIP_CHECK(hipEventRecord(g_slot_ev[slot], g_ctx.stream));
+}
+
+void read_back_wait(int slot, void *host, size_t bytes) {
+  ISVD_CHECK(slot >= 0 && slot < kSlots && bytes <= kSlotBytes, "read_back_wait: bad slot");
+  spin_on(g_slot_ev[slot]);
+  std::memcpy(host, slot_ptr(slot), bytes);
 }
 
 bool is_device_pointer(const void *ptr) {
@@ -984,13 +1037,22 @@ void isvd_Init_gpu(int *, char ***, const isvd_MpiComm) {
 void isvd_Finalize_gpu(void) {
   guarded([] {
     if (g_init) {
+      // everything queued on either stream is done before anything is released; events and
+      // pinned memory go before the streams they were recorded or copied on
+      HIP_CHECK(hipStreamSynchronize(g_ctx.aux));
       HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
       if (g_ws) HIP_CHECK(hipFreeAsync(g_ws, g_ctx.stream));
       HIP_CHECK(hipStreamSynchronize(g_ctx.stream));
       g_ws = nullptr;
       g_ws_bytes = 0;
+      release_pinned();
+      HIP_CHECK(hipEventDestroy(g_ctx.fork));
+      HIP_CHECK(hipEventDestroy(g_ctx.join));
+      HIP_CHECK(hipStreamDestroy(g_ctx.aux));
       HIP_CHECK(hipStreamDestroy(g_ctx.stream));
       g_ctx.stream = nullptr;
+      g_ctx.aux = nullptr;
+      g_ctx.fork = g_ctx.join = nullptr;
       hipMemPool_t pool = nullptr;
       HIP_CHECK(hipDeviceGetDefaultMemPool(&pool, 0));
       HIP_CHECK(hipMemPoolTrimTo(pool, 0));

@@ -69,6 +69,10 @@ struct Arena {
 
 struct Ctx {
   hipStream_t stream = nullptr;
+  // a second stream (default priority) for a product that runs beside work on `stream`; it
+  // is joined to `stream` by the events before anything else reads what it wrote
+  hipStream_t aux = nullptr;
+  hipEvent_t fork = nullptr, join = nullptr;
   int device = 0;
   Arena *arena = nullptr;  // non-null while a no-allocation entry point runs
 };
@@ -128,6 +132,14 @@ size_t splitk_budget();
 // Host <- device copy of a few values that returns as soon as they land: the copy goes to
 // pinned host memory and the host spins on an event instead of a blocking stream wait.
 void read_back(void *host, const void *dev, size_t bytes);
+// Pinned slots (0..3) for values a kernel hands to the host while the host queues more work:
+// pinned_slot gives the device address of a slot (a kernel may store up to 256 bytes there;
+// they are seen by the host once the kernel is done), pinned_slot_record records the slot's
+// event on the stream after that kernel, and read_back_wait spins until that event and copies
+// the bytes out.  A slot holds one pending value at a time.
+void *pinned_slot(int slot);
+void pinned_slot_record(int slot);
+void read_back_wait(int slot, void *host, size_t bytes);
 
 // True when ptr is device (HBM) memory usable in place by kernels on the current device.
 bool is_device_pointer(const void *ptr);

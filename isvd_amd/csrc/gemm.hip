@@ -642,6 +642,39 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
   }
 }
 
+// 64 x 64 tiles with a given number of K chunks and no other choice made: the caller either
+// wants the raw partials (ksplit > 1, written to g.W) or a product whose short K needs no split.
+// Returns the number of chunks actually used (K is cut into multiples of the K tile).
+template <typename T, bool TA, bool TB>
+int launch_fixed64(GemmArgs<T> g, i64 batch, int ksplit) {
+  constexpr int BM = 64, BN = 64, BK = 16;
+  const i64 tiles = ((g.m + BM - 1) / BM) * ((g.n + BN - 1) / BN);
+  if (ksplit < 1) ksplit = 1;
+  i64 kchunk = (g.k + ksplit - 1) / ksplit;
+  kchunk = (kchunk + BK - 1) / BK * BK;
+  ksplit = (int)((g.k + kchunk - 1) / kchunk);
+  if (ksplit < 1) ksplit = 1;
+  g.ksplit = ksplit;
+  g.kchunk = (ksplit == 1) ? (g.k > 0 ? g.k : 1) : kchunk;
+  g.group_m = 8;
+  ISVD_CHECK(batch * ksplit <= 65535, "grid.y too large (batch %lld x ksplit %d)", batch, ksplit);
+  hipLaunchKernelGGL((gemm_kernel<T, TA, TB, BM, BN, BK, 2, 2>),
+                     dim3((unsigned)tiles, (unsigned)(batch * ksplit)), dim3(256), 0,
+                     ctx().stream, g);
+  LAUNCH_CHECK();
+  return ksplit;
+}
+
+template <typename T>
+int fixed64(char transa, char transb, GemmArgs<T> g, i64 batch, int ksplit) {
+  const bool ta = (transa == 'T' || transa == 't');
+  const bool tb = (transb == 'T' || transb == 't');
+  if (!ta && !tb) return launch_fixed64<T, false, false>(g, batch, ksplit);
+  if (!ta && tb) return launch_fixed64<T, false, true>(g, batch, ksplit);
+  if (ta && !tb) return launch_fixed64<T, true, false>(g, batch, ksplit);
+  return launch_fixed64<T, true, true>(g, batch, ksplit);
+}
+
 template <typename T, bool TA, bool TB>
 void launch(GemmArgs<T> g, i64 batch) {
   const bool big = (g.m >= 128 && g.n >= 128);
@@ -700,6 +733,40 @@ void gemm_ct(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A,
              const T *B, i64 ldb, T beta, T *C, i64 ldc) {
   gemm_impl<T>(transa, transb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, 1, 0, 0, 0, 0, 1);
 }
+
+template <typename T>
+void gemm_shortk(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda,
+                 const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch, i64 sA, i64 sB, i64 sC) {
+  if (m <= 0 || n <= 0 || batch <= 0) return;
+  ISVD_CHECK(k > 0, "gemm_shortk: k = %lld", k);
+  GemmArgs<T> g{};
+  g.m = m; g.n = n; g.k = k; g.alpha = alpha; g.beta = beta;
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb; g.C = C; g.ldc = ldc;
+  g.sA = sA; g.sB = sB; g.sC = sC;
+  (void)fixed64<T>(transa, transb, g, batch, 1);
+}
+
+template <typename T>
+int gemm_partials(char transa, char transb, i64 m, i64 n, i64 k, const T *A, i64 lda,
+                  const T *B, i64 ldb, T *W, int ksplit) {
+  ISVD_CHECK(m > 0 && n > 0 && k > 0, "gemm_partials: empty product");
+  GemmArgs<T> g{};
+  g.m = m; g.n = n; g.k = k; g.alpha = T(1); g.beta = T(0);
+  g.A = A; g.lda = lda; g.B = B; g.ldb = ldb;
+  g.C = W; g.ldc = m;  // one chunk: the product itself, dense
+  g.W = W;
+  return fixed64<T>(transa, transb, g, 1, ksplit);
+}
+
+template void gemm_shortk<double>(char, char, i64, i64, i64, double, const double *, i64,
+                                  const double *, i64, double, double *, i64, i64, i64, i64,
+                                  i64);
+template void gemm_shortk<float>(char, char, i64, i64, i64, float, const float *, i64,
+                                 const float *, i64, float, float *, i64, i64, i64, i64, i64);
+template int gemm_partials<double>(char, char, i64, i64, i64, const double *, i64,
+                                   const double *, i64, double *, int);
+template int gemm_partials<float>(char, char, i64, i64, i64, const float *, i64, const float *,
+                                  i64, float *, int);
 
 template <typename T>
 void syrk_full(char trans, i64 n, i64 k, T alpha, const T *A, i64 lda, T *C, i64 ldc) {

@@ -17,6 +17,20 @@ void gemm(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i6
           const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch = 1, i64 sA = 0, i64 sB = 0,
           i64 sC = 0);
 
+// The same product on 64 x 64 tiles without split-K: for a short K (<= a few hundred) whose
+// output alone fills the device, where a split would only add a reduction pass.
+template <typename T>
+void gemm_shortk(char transa, char transb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda,
+                 const T *B, i64 ldb, T beta, T *C, i64 ldc, i64 batch = 1, i64 sA = 0,
+                 i64 sB = 0, i64 sC = 0);
+
+// Raw split-K partial sums of op(A) op(B) (64 x 64 tiles): W[s] (m x n, dense column-major)
+// is the product over the s-th of about ksplit K chunks.  Returns the number of chunks written;
+// the caller sums them (in a fixed order) where it needs the product.
+template <typename T>
+int gemm_partials(char transa, char transb, i64 m, i64 n, i64 k, const T *A, i64 lda,
+                  const T *B, i64 ldb, T *W, int ksplit);
+
 // C^T := alpha * (op(A) * op(B))^T + beta * C^T: the same product stored transposed, i.e. C
 // is n x m with leading dimension ldc >= n.  (M B for a symmetric M is (B^T M)^T, which takes
 // the wide-tile kernel when B has few columns.)
@@ -66,16 +80,68 @@ template <typename T>
 void small_gemm(char ta, char tb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda, const T *B,
                 i64 ldb, T beta, T *C, i64 ldc, T diag = T(0));
 
-// C = (I/2 + (I/4 - Z)^{1/2})^{1/2} and C^-1 (n x n, ld n) by coupled Newton-Schulz square
-// roots; valid while ||4 Z||_2 < 1.  work: 6 n*n entries.  err (device, may be null):
-// err[0] = ||C - I||_F.
+// Newton-Schulz step counts for the two square roots of kn_chain from
+// rho >= the spectral radius of 4 Z (the iterates are polynomials in the symmetric Z, so the
+// residual bound holds eigenvalue by eigenvalue).  The coupled iteration on A = I - R has
+// residual R_k = I - W_k Y_k with ||R_{k+1}|| <= ||R_k||^2 (3 + ||R_k||) / 4; stage 1 starts
+// from ||R_0|| <= rho, stage 2 from ||I - (I + P)/2|| = max |1 - sqrt(1 - x)| / 2 over
+// |x| <= rho, which is <= rho / (2 (1 + sqrt(1 - rho))).  Each stage runs until the bound
+// falls below the unit roundoff; false when that takes more than kMaxNs steps (rho too close
+// to 1: the caller then uses the eigendecomposition).
+constexpr int kMaxNs = 20;
 template <typename T>
-void kn_matfun(i64 n, const T *Z, T *C, T *Cinv, T *work, int iters1, int iters2,
-               T *err = nullptr);
+__host__ __device__ inline bool ns_counts(double rho, int *k1, int *k2) {
+  const double eps = (sizeof(T) == 8) ? 0x1p-53 : 0x1p-24;
+  if (!(rho < 1.0)) return false;
+  bool ok = true;
+  double r = rho;
+  for (int stage = 0; stage < 2; ++stage) {
+    int k = 0;
+    while (r > eps && k < kMaxNs) {
+      r = r * r * (3.0 + r) / 4.0;
+      ++k;
+    }
+    if (r > eps) ok = false;
+    *(stage == 0 ? k1 : k2) = k < 1 ? 1 : k;
+    r = rho / (2.0 * (1.0 + sqrt(1.0 - rho)));
+  }
+  return ok;
+}
 
-// out[0] = ||A - c I||_F  (n x n, ld n)
+// One launch for the l x l part of a Kolmogorov-Nagumo iteration (l <= 128), from the partial
+// sums of [D | Dg] (l x 2l):
+//   D | Dg = sum of the partials / N;  Z = Dg - D^T D;
+//   C = f(Z) = (I/2 + (I/4 - Z)^(1/2))^(1/2) and C^-1: the Taylor series of f and 1/f when
+//   4 ||Z^2||_F^(1/2) is small enough for a series of degree <= 31 (Paterson-Stockmeyer), else
+//   coupled Newton-Schulz square roots with step counts from rho = 4 ||Z^4||_F^(1/4)
+//   (ns_counts);  F = C - D C^-1;
+//   W3 = [F; C^-1 / N], [F; C^-1], [F; C^-1]  (three 2l x l blocks, ld 2l, block stride 2 l^2).
+// scal: [0] ||C - I||_F, [1] ||Z^4||_F (0 on the series path), [2] the rho used, [3] status:
+// 0 done, 1 rho out of reach of both (or mode 1): nothing after rho is written and the caller
+// takes the eigendecomposition of Z; 2 a grid barrier timed out (an error).
+// mode: 0 automatic, 1 eigendecomposition, 2 Newton-Schulz (never the series).
+// The workgroups meet at grid barriers (one per dependent product); every wait is bounded.
+// KnChainWork holds the intermediates (Z is its first l x l block), the series coefficients
+// and the barrier state; kn_chain_init prepares it once per integration.
 template <typename T>
-void frob_shift(i64 n, const T *A, T c, T *out);
+struct KnChainWork {
+  T *buf = nullptr;                   // kn_chain_work_size(l) entries
+  unsigned long long *bar = nullptr;  // kKnChainBarWords words
+  T *Z() const { return buf; }
+};
+constexpr int kKnChainPart = 256;
+constexpr int kKnChainCoef = 66;
+constexpr int kKnChainBarWords = 8;
+inline i64 kn_chain_work_size(i64 l) { return 9 * l * l + kKnChainPart + kKnChainCoef; }
+// parts: nparts raw partial sums of B^T [B | Bg] (l x 2l each, dense, as gemm_partials writes
+// them); their sum / N is written to DDg (D | Dg, ld l) before the chain starts.
+template <typename T>
+void kn_chain(int l, T ninv, const T *parts, int nparts, T *DDg, const KnChainWork<T> &w,
+              T *W3, T *scal, int mode);
+template <typename T>
+void kn_chain_init(const KnChainWork<T> &w, int l);
+void zero_u64(unsigned long long *p, int n);
+
 
 // In-place inverse (Gauss-Jordan, partial pivoting), n <= 128; *info_dev = 0 or the 1-based
 // index of a zero pivot.

@@ -290,129 +290,174 @@ void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst) {
 //   Bg = Qs^T (Qs B / N) = M B / N with M = Qs^T Qs (Nl x Nl, formed once),
 //   X_+ = X_c F + (B_c / N) C^-1   (the image of Q_+ = Q_c F + G_c C^-1),
 // so each iteration touches only Nl x l and l x l matrices; Qbar = Qs X is formed at the end.
-// Newton-Schulz step counts for kn_matfun from rho >= the spectral radius of 4 Z (the iterates
-// are polynomials in the symmetric Z, so the residual bound holds eigenvalue by eigenvalue).
-// The coupled iteration on A = I - R has residual R_k = I - W_k Y_k with
-// ||R_{k+1}|| <= ||R_k||^2 (3 + ||R_k||) / 4; stage 1 starts from ||R_0|| <= rho, stage 2 from
-// ||I - (I + P)/2|| = max |1 - sqrt(1 - x)| / 2 over |x| <= rho <= rho / (2 (1 + sqrt(1 - rho))).
-// Each stage runs until the bound falls below the unit roundoff; false when that takes more
-// than kMaxNs steps (rho too close to 1: the caller then uses the eigendecomposition).
-constexpr int kMaxNs = 20;
-template <typename T>
-bool ns_counts(double rho, int *k1, int *k2) {
-  const double eps = (sizeof(T) == 8) ? 0x1p-53 : 0x1p-24;
-  bool ok = true;
-  auto steps = [&](double r) {
-    int k = 0;
-    while (r > eps && k < kMaxNs) {
-      r = r * r * (3.0 + r) / 4.0;
-      ++k;
-    }
-    if (r > eps) ok = false;
-    return k < 1 ? 1 : k;
-  };
-  if (!(rho < 1.0)) return false;
-  *k1 = steps(rho);
-  *k2 = steps(rho / (2.0 * (1.0 + std::sqrt(1.0 - rho))));
-  return ok;
-}
-
-// [W1 | W2] (2l x l each, ld 2l) := [F; C^-1 / N] and [F; C^-1]
+// W3 := [F; C^-1 / N], [F; C^-1], [F; C^-1]  (three 2l x l blocks, ld 2l)
 template <typename T>
 __global__ void kn_assemble_kernel(int l, const T *F, const T *Ci, T ninv, T *W) {
+  const i64 blk = (i64)2 * l * l;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < l * l; e += gridDim.x * blockDim.x) {
     const int i = e % l, j = e / l;
     const T f = F[e], ci = Ci[e];
-    T *w1 = W + (i64)j * 2 * l, *w2 = W + (i64)2 * l * l + (i64)j * 2 * l;
-    w1[i] = f;
-    w1[l + i] = ninv * ci;
-    w2[i] = f;
-    w2[l + i] = ci;
+    T *w = W + (i64)j * 2 * l;
+    w[i] = f;
+    w[l + i] = ninv * ci;
+    w[blk + i] = f;
+    w[blk + l + i] = ci;
+    w[2 * blk + i] = f;
+    w[2 * blk + l + i] = ci;
   }
 }
 
-// The KN iteration in sketch coordinates.  Per iteration (one host synchronisation):
-//   [X | B | Bg] (Nl x 3l) holds X_c, B_c = Qs^T Q_c and Bg_c = M B_c / N;
-//   [D | Dg] = B_c^T [B_c | Bg_c] / N (one product), Z = Dg - D^T D, rho = 4 ||Z^4||_F^(1/4);
-//   C, C^-1 from Z (Newton-Schulz with step counts chosen from rho, or the eigendecomposition
-//   when rho is too close to 1 for kMaxNs steps);  F = C - D C^-1;
-//   [X_+ | B_+] = [X_c B_c] [F; C^-1/N] and [B_c Bg_c] [F; C^-1] (one batched product);
-//   then Bg, D, Dg, Z and rho of the next iterate are formed before the host reads err and
-//   the next rho together (the work after the last iteration is discarded).
+// Runs the enclosed launches on another stream instead of ctx().stream.
+struct OnStream {
+  hipStream_t saved;
+  explicit OnStream(hipStream_t s) : saved(ctx().stream) { ctx().stream = s; }
+  ~OnStream() { ctx().stream = saved; }
+};
+
+// At scope exit (also when an error unwinds it) the first stream waits for everything queued on
+// aux, so that buffers freed in stream order on the first are no longer in use.
+struct JoinAux {
+  ~JoinAux() {
+    if (hipEventRecord(ctx().join, ctx().aux) == hipSuccess)
+      (void)hipStreamWaitEvent(ctx().stream, ctx().join, 0);
+  }
+};
+
+// The way kn_chain takes C = f(Z): ISVD_KN_EIG=1 the eigendecomposition of Z in every KN
+// iteration, ISVD_KN_NS=1 Newton-Schulz whenever it converges (the paths otherwise taken only
+// when rho is too large for the series, or for Newton-Schulz); default automatic.
+int kn_mode() {
+  const char *e = std::getenv("ISVD_KN_EIG");
+  if (e && e[0] == '1') return 1;
+  const char *n = std::getenv("ISVD_KN_NS");
+  if (n && n[0] == '1') return 2;
+  return 0;
+}
+
+// The KN iteration in sketch coordinates.  A state [X | B | Bg | G] (Nl x 4l) holds X_c,
+// B_c = Qs^T Q_c, Bg_c = M B_c / N and G_c = M Bg_c / N; [D | Dg] = B_c^T [B_c | Bg_c] / N.
+// Iteration it, from state s to state s+1 (three states in a ring):
+//   second stream: G_c := M Bg_c / N (waits for Bg_c only);
+//   kn_chain: [D | Dg] from the Gram partials, Z = Dg - D^T D, rho, C and C^-1, F = C - D C^-1,
+//     W3 = [F; C^-1/N], [F; C^-1], [F; C^-1];
+//   [X_+ | B_+] := [X_c B_c | B_c Bg_c] W3 (beside G_c), then, once G_c is there,
+//   Bg_+ := [Bg_c G_c] W3 (B_+ and Bg_+ are M X_+ and M B_+ / N because M X_c = B_c and
+//   M Bg_c / N = G_c), and the Gram partials of the new state;
+//   err = ||C - I||_F copied to a pinned slot.
+// The host queues iteration it + 1 before it waits for err of iteration it, so the GPU does not
+// idle between iterations; the work queued past the last iteration is discarded.  When rho is
+// out of reach of the chain (status 1), iteration it is done again from its state (intact in the
+// ring) with C and C^-1 from the eigendecomposition of Z.
 template <typename T>
 void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out, i64 *iter_out,
                          double *err_out) {
-  const i64 l = d.l, Nl = d.Nl;
+  const i64 l = d.l, Nl = d.Nl, ll = l * l;
   const T ninv = T(1) / T(d.N);
-  DBuf<T> xbuf(2 * Nl * 3 * l), dz(2 * l * l), w(l), zv(l * l), zs(l * l), zi(l * l), c(l * l),
-      cinv(l * l), wmat(4 * l * l), scal(2), work(6 * l * l);
-  T *xb[2] = {xbuf.get(), xbuf.get() + Nl * 3 * l};
-  T *D = dz.get(), *Z = dz.get() + l * l;  // [D | Dg]; Dg becomes Z in place
-  // Bg, [D | Dg], Z and ||Z^4||_F (scal[1]) of the iterate held in xb[q]
-  auto prep = [&](int q) {
-    T *X = xb[q], *B = X + Nl * l, *Bg = X + 2 * Nl * l;
-    (void)X;
-    // Bg := M B / N  (:174-178), as (B^T M)^T: M is exactly symmetric
-    gemm_ct<T>('T', 'N', l, Nl, Nl, ninv, B, Nl, M, Nl, T(0), Bg, Nl);
-    // D := B^T B / N (:181), Dg := B^T Bg / N (:184-186)
-    gemm<T>('T', 'N', l, 2 * l, Nl, ninv, B, Nl, B, Nl, T(0), D, l);
-    // Z := Dg - D^T D (:187-190)
-    small_gemm<T>('T', 'N', l, l, l, T(-1), D, l, D, l, T(1), Z, l);
-    // ||Z^4||_F^(1/4) = (sum lambda^8)^(1/8) bounds the spectral radius of the symmetric Z
-    // (up to l^(3/8) times tighter than ||Z||_F)
-    T *z2 = work.get(), *z4 = work.get() + l * l;
-    small_gemm<T>('N', 'N', l, l, l, T(1), Z, l, Z, l, T(0), z2, l);
-    small_gemm<T>('N', 'N', l, l, l, T(1), z2, l, z2, l, T(0), z4, l);
-    frob_shift<T>(l, z4, T(0), scal.get() + 1);
+  const int mode = kn_mode();
+  // [D | Dg] as K-chunk partial sums over about 256 workgroups of 64 x 64 tiles
+  const i64 dtiles = ((l + 63) / 64) * ((2 * l + 63) / 64);
+  int want = (int)(256 / dtiles);
+  if (want < 1) want = 1;
+  if (want > Nl / 64) want = (int)(Nl / 64 > 0 ? Nl / 64 : 1);
+  const i64 sz = Nl * 4 * l;
+  DBuf<T> sbuf(3 * sz), ddg(2 * ll), w3(6 * ll), scal(4), parts((i64)(want + 1) * 2 * ll);
+  DBuf<T> cwork(kn_chain_work_size(l));
+  DBuf<unsigned long long> bar(kKnChainBarWords);
+  KnChainWork<T> chain;
+  chain.buf = cwork.get();
+  chain.bar = bar.get();
+  kn_chain_init<T>(chain, (int)l);
+  // eigendecomposition path
+  DBuf<T> w(l), zv(ll), zs(ll), zi(ll), c(ll), cinv(ll);
+  const JoinAux join_aux;  // destroyed before the buffers above
+  auto X = [&](i64 q) { return sbuf.get() + (q % 3) * sz; };
+  auto B = [&](i64 q) { return X(q) + Nl * l; };
+  auto Bg = [&](i64 q) { return X(q) + 2 * Nl * l; };
+  auto G = [&](i64 q) { return X(q) + 3 * Nl * l; };
+  // D := B^T B / N (:181), Dg := B^T Bg / N (:184-186): the K-chunk partials, summed by kn_chain
+  int nparts = 0;
+  auto gram = [&](i64 q) {
+    nparts = gemm_partials<T>('T', 'N', l, 2 * l, Nl, B(q), Nl, B(q), Nl, parts.get(), want);
   };
-  // X_c := [I; 0] (Qc := Q_0, :156);  B_c := Qs^T Q_0 = M[:, 0:l]  (:159-160)
-  fill<T>(Nl, l, T(0), xb[0], Nl, T(1));
-  omatcopy<T>('N', Nl, l, T(1), M, Nl, xb[0] + Nl * l, Nl);
-  prep(0);
-  T h[2];
-  read_back(&h[1], scal.get() + 1, sizeof(T));
-  int cur = 0;
+  // G_q := M Bg_q / N on the second stream, after the fork event (recorded once Bg_q is written)
+  auto g_product = [&](i64 q) {
+    HIP_CHECK(hipStreamWaitEvent(ctx().aux, ctx().fork, 0));
+    {
+      OnStream on(ctx().aux);
+      gemm_ct<T>('T', 'N', l, Nl, Nl, ninv, Bg(q), Nl, M, Nl, T(0), G(q), Nl);
+    }
+    HIP_CHECK(hipEventRecord(ctx().join, ctx().aux));
+  };
+  // X_+ := X_c F + (B_c / N) C^-1 (image of :221-222); B_+ := B_c F + Bg_c C^-1 (:225-226);
+  // then Bg_+ := Bg_c F + G_c C^-1 after the join; the fork for the next G; the next Gram
+  auto updates = [&](i64 q) {
+    gemm_shortk<T>('N', 'N', Nl, l, 2 * l, T(1), X(q), Nl, w3.get(), 2 * l, T(0), X(q + 1), Nl,
+                   2, Nl * l, 2 * ll, Nl * l);
+    HIP_CHECK(hipStreamWaitEvent(ctx().stream, ctx().join, 0));
+    gemm_shortk<T>('N', 'N', Nl, l, 2 * l, T(1), Bg(q), Nl, w3.get() + 4 * ll, 2 * l, T(0),
+                   Bg(q + 1), Nl);
+    HIP_CHECK(hipEventRecord(ctx().fork, ctx().stream));
+    gram(q + 1);
+  };
+  // iteration it from state it - 1 (queued; the chain stores err, rho and its status straight
+  // into pinned slot it % 2, ready for the host as soon as the chain is done)
+  T *slot[2] = {static_cast<T *>(pinned_slot(0)), static_cast<T *>(pinned_slot(1))};
+  auto enqueue = [&](i64 it, int m) {
+    const i64 q = it - 1;
+    g_product(q);
+    // [D | Dg]; C and C^-1 (:193-212): C = (I/2 + (I/4 - Z)^{1/2})^{1/2}; F (:218); W3
+    kn_chain<T>((int)l, ninv, parts.get(), nparts, ddg.get(), chain, w3.get(), slot[it & 1], m);
+    pinned_slot_record((int)(it & 1));
+    updates(q);
+  };
+  // X_c := [I; 0] (Qc := Q_0, :156);  B_c := Qs^T Q_0 = M[:, 0:l]  (:159-160);
+  // Bg_c := M B_c / N (:174-178), as (B^T M)^T: M is exactly symmetric
+  fill<T>(Nl, l, T(0), X(0), Nl, T(1));
+  omatcopy<T>('N', Nl, l, T(1), M, Nl, B(0), Nl);
+  gemm_ct<T>('T', 'N', l, Nl, Nl, ninv, B(0), Nl, M, Nl, T(0), Bg(0), Nl);
+  HIP_CHECK(hipEventRecord(ctx().fork, ctx().stream));
+  gram(0);
+  enqueue(1, mode);
   i64 it;
   double error = -INFINITY;
+  T h[4];
   for (it = 1;; ++it) {
-    const double rho = 4.0 * std::sqrt(std::sqrt((double)h[1]));
-    // C and inv(C) (:193-212): C = (I/2 + (I/4 - Z)^{1/2})^{1/2} = V diag(s) V^T with
-    // s = sqrt(1/2 + sqrt(1/4 - eig(Z)))
-    // Newton-Schulz while it converges within kMaxNs steps per stage (rho < ~0.999), else
-    // the eigendecomposition
-    int k1 = 0, k2 = 0;
-    const bool use_eig = !ns_counts<T>(rho, &k1, &k2);
-    if (!use_eig) {
-      // and ||C - I||_F = ||s - 1||_2 into scal[0]
-      kn_matfun<T>(l, Z, c.get(), cinv.get(), work.get(), k1, k2, scal.get());
-    } else {
-      syev_small<T>(l, Z, l, w.get(), zv.get(), l);
+    const bool more = it < args.maxit;
+    if (more) enqueue(it + 1, mode);
+    read_back_wait((int)(it & 1), h, 4 * sizeof(T));
+    ISVD_CHECK(h[3] != T(2), "KN: a grid barrier of the l x l chain timed out");
+    const bool use_eig = h[3] == T(1);
+    if (use_eig) {
+      // iteration it again from its state, with C = V diag(s) V^T,
+      // s = sqrt(1/2 + sqrt(1/4 - eig(Z))) and err = ||s - 1||_2 (G_c of the state is valid);
+      // everything queued so far is finished first
+      HIP_CHECK(hipStreamSynchronize(ctx().aux));
+      const i64 q = it - 1;
+      gram(q);
+      kn_chain<T>((int)l, ninv, parts.get(), nparts, ddg.get(), chain, w3.get(), scal.get(), 1);
+      syev_small<T>(l, chain.Z(), l, w.get(), zv.get(), l);
       hipLaunchKernelGGL(kn_scalars_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
                          (const T *)w.get(), (const T *)zv.get(), zs.get(), zi.get(), scal.get());
       LAUNCH_CHECK();
       small_gemm<T>('N', 'T', l, l, l, T(1), zs.get(), l, zs.get(), l, T(0), c.get(), l);
       small_gemm<T>('N', 'T', l, l, l, T(1), zi.get(), l, zi.get(), l, T(0), cinv.get(), l);
-      hipLaunchKernelGGL(sym_from_upper_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
+      hipLaunchKernelGGL(sym_from_upper_kernel<T>, dim3(grid_for(ll)), dim3(256), 0,
                          ctx().stream, (int)l, cinv.get(), l);
       LAUNCH_CHECK();
+      // F [in C] := C - D inv(C)  (:218)
+      small_gemm<T>('N', 'N', l, l, l, T(-1), ddg.get(), l, cinv.get(), l, T(1), c.get(), l);
+      hipLaunchKernelGGL(kn_assemble_kernel<T>, dim3(grid_for(ll)), dim3(256), 0, ctx().stream,
+                         (int)l, (const T *)c.get(), (const T *)cinv.get(), ninv, w3.get());
+      LAUNCH_CHECK();
+      HIP_CHECK(hipEventRecord(ctx().join, ctx().stream));  // G_c is already in the state
+      updates(q);
+      read_back(h, scal.get(), sizeof(T));
+      if (more) enqueue(it + 1, mode);  // the queued one ran from a discarded state
     }
-    // F [in C] := C - D inv(C)  (:218)
-    small_gemm<T>('N', 'N', l, l, l, T(-1), D, l, cinv.get(), l, T(1), c.get(), l);
-    hipLaunchKernelGGL(kn_assemble_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
-                       (int)l, (const T *)c.get(), (const T *)cinv.get(), ninv, wmat.get());
-    LAUNCH_CHECK();
-    // X_+ := X_c F + (B_c / N) inv(C) (image of :221-222);  B_+ := B_c F + Bg_c inv(C) (:225-226)
-    const int nxt = cur ^ 1;
-    gemm<T>('N', 'N', Nl, l, 2 * l, T(1), xb[cur], Nl, wmat.get(), 2 * l, T(0), xb[nxt], Nl, 2,
-            Nl * l, 2 * l * l, Nl * l);
-    // the next iterate's Bg, D, Z (needed unless this iteration is the last)
-    prep(nxt);
-    read_back(h, scal.get(), 2 * sizeof(T));
     error = (double)h[0];
     if (trace_on())
-      std::fprintf(stderr, "isvd KN iter %lld err %.17g rho %.3g (%s)\n", it, error, rho,
-                   use_eig ? "eig" : "newton-schulz");
-    cur = nxt;
+      std::fprintf(stderr, "isvd KN iter %lld err %.17g rho %.3g (%s)\n", it, error,
+                   (double)h[2], use_eig ? "eig" : (h[1] == T(0) ? "series" : "newton-schulz"));
     // convergence (:230-241)
     if (error <= args.tol) break;
     if (it == args.maxit) {
@@ -420,7 +465,10 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
       break;
     }
   }
-  omatcopy<T>('N', Nl, l, T(1), xb[cur], Nl, x_out, Nl);
+  // the result is the state after the last iteration; the queued iteration past it writes the
+  // next state of the ring only
+  const i64 last = (error <= args.tol) ? it : it - 1;
+  omatcopy<T>('N', Nl, l, T(1), X(last), Nl, x_out, Nl);
   *iter_out = it;
   *err_out = error;
 }

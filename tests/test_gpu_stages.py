@@ -79,6 +79,39 @@ def test_kn_golden(golden, torch_cuda, dtype):
     assert np.abs(host(q) - golden["q_kn"]).max() < TOL[dtype]
 
 
+# C = (I/2 + (I/4 - Z)^(1/2))^(1/2) and C^-1 are taken from a Taylor series of Z when rho is
+# small, by Newton-Schulz otherwise, and from the eigendecomposition of Z when rho is out of
+# reach of both.  ISVD_KN_NS=1 keeps the series out, ISVD_KN_EIG=1 takes the eigendecomposition
+# in every iteration: each path must reproduce the reference's vectors and iteration counts.
+KN_PATHS = ["ISVD_KN_NS", "ISVD_KN_EIG"]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("path", KN_PATHS)
+def test_kn_golden_other_paths(golden, torch_cuda, dtype, path, monkeypatch):
+    monkeypatch.setenv(path, "1")
+    p = capi.create_param(101, 101, 11, 0, 12)
+    q, it, err = pipeline.integrate(p, dev(torch_cuda, golden["qs"], dtype), "KN", [256, 1e-4])
+    if dtype == np.float64:
+        assert it == 81
+        assert err <= 1e-4
+    assert np.abs(host(q) - golden["q_kn"]).max() < TOL[dtype]
+
+
+@pytest.mark.parametrize("path", KN_PATHS)
+def test_kn_l128_other_paths_match_oracle(torch_cuda, path, monkeypatch):
+    from conftest import lowrank_matrix
+    monkeypatch.setenv(path, "1")
+    m, n, k, p, N = 4096, 1024, 100, 28, 4
+    a = lowrank_matrix(m, n, 1.0 / np.arange(1, n + 1), seed=5, noise=1e-6)
+    qs = O.orthogonalize_tall_skinny_qr(a @ np.random.default_rng(6).standard_normal((n, 512)), N)
+    q_ref, it_ref, _ = O.integrate_kolmogorov_nagumo(qs, N)
+    q, it, _ = pipeline.integrate(capi.create_param(m, n, k, p, N), dev(torch_cuda, qs, np.float64),
+                                  "KN")
+    assert it == it_ref
+    assert np.abs(host(q) - q_ref).max() < 1e-10
+
+
 def test_wy_golden(golden, torch_cuda):
     # single-precision WY is excluded by the reference too (check/CMakeLists.txt:86-88)
     p = capi.create_param(101, 101, 11, 0, 12)
