@@ -86,17 +86,15 @@ void stage_orth_tsqr_b(const Dims &d, float *yst, i64 ldyst) {
   stage_orth_tsqr<float>(d, yst, ldyst);
 }
 
-// @x@_integrate_kolmogorov_nagumo.c:155-253
+// @x@_integrate_kolmogorov_nagumo.c:155-253.  The iteration runs in FP32 on the FP32-accurate
+// M (split-BF16 x3 products): FP64 arithmetic on it would carry digits M does not have.
 void stage_integrate_kn_b(const Dims &d, const KnArgs &args, const float *yst, i64 ldyst,
                           float *qt, i64 ldqt, i64 *iter, double *err) {
   const i64 l = d.l, Nl = d.Nl;
   DBuf<float> m32(Nl * Nl), x32(Nl * l);
-  DBuf<double> m64(Nl * Nl), x64(Nl * l);
   gram_split(d, yst, ldyst, m32.get());
-  convert<double, float>(Nl, Nl, m32.get(), Nl, m64.get(), Nl);
+  integrate_kn_coords<float>(d, args, m32.get(), x32.get(), iter, err);
   m32.release();
-  integrate_kn_coords<double>(d, args, m64.get(), x64.get(), iter, err);
-  convert<float, double>(Nl, l, x64.get(), Nl, x32.get(), Nl);
   gemm<float>('T', 'N', l, d.m, Nl, 1.f, x32.get(), Nl, yst, ldyst, 0.f, qt, ldqt);
 }
 
