@@ -67,6 +67,58 @@ __global__ void f32_to_bf16_kernel(i64 count, const float *src, uint16_t *dst) {
     dst[e] = (uint16_t)bf16_rne(src[e]);
 }
 
+// bop (2n x 3l, ld 2n) from B (n x l): B = b0 + b1 + b2, its BF16 terms (exact for FP32), laid
+// out as the column blocks [b0; b0], [b1; b1], [b2; 0], so that
+// [Mh | Ml] bop = [Mh b0 + Ml b0 | Mh b1 + Ml b1 | Mh b2]
+__global__ void split_sym_rhs_kernel(i64 n, i64 l, const float *B, i64 ldb, uint16_t *bop) {
+  const i64 ld = 2 * n;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < n * l;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 i = e % n, j = e / n;
+    float r = B[i + j * ldb];
+    const uint32_t h0 = bf16_rne(r);
+    r -= bf16_val(h0);
+    const uint32_t h1 = bf16_rne(r);
+    r -= bf16_val(h1);
+    const uint32_t h2 = bf16_rne(r);
+    uint16_t *c0 = bop + j * ld, *c1 = bop + (l + j) * ld, *c2 = bop + (2 * l + j) * ld;
+    c0[i] = (uint16_t)h0;
+    c0[n + i] = (uint16_t)h0;
+    c1[i] = (uint16_t)h1;
+    c1[n + i] = (uint16_t)h1;
+    c2[i] = (uint16_t)h2;
+    c2[n + i] = 0;
+  }
+}
+
+// G(i, j) := alpha * the sum over chunks s = 0, 1, ... and, within a chunk, over the column
+// blocks c = 0, 1, 2 of W[s](i, c l + j) (W[s] n x 3l dense), in that fixed order
+__global__ void split_sym_fold_kernel(i64 n, i64 l, int parts, const float *W, float alpha,
+                                      float *G, i64 ldg) {
+  const i64 blk = n * l;
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < blk;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 i = e % n, j = e / n;
+    const float *w = W + i + j * n;
+    float v = 0.f;
+    for (int q = 0; q < parts; ++q, w += 3 * blk) {
+      v += w[0];
+      v += w[blk];
+      v += w[2 * blk];
+    }
+    G[i + j * ldg] = alpha * v;
+  }
+}
+
+// B(i, j) := Mh(i, j) + Ml(i, j) (exact in FP32) for S = [Mh; Ml] (2n x n, ld 2n), j < l
+__global__ void split_sym_columns_kernel(i64 n, i64 l, const uint16_t *S, float *B, i64 ldb) {
+  for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < n * l;
+       e += (i64)gridDim.x * blockDim.x) {
+    const i64 i = e % n, j = e / n;
+    B[i + j * ldb] = bf16_val(S[i + j * 2 * n]) + bf16_val(S[n + i + j * 2 * n]);
+  }
+}
+
 int grid_of(i64 n) {
   i64 b = (n + 255) / 256;
   return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
@@ -113,6 +165,30 @@ void gram_combine(i64 n, const float *C, i64 ldc, float *M, i64 ldm) {
   if (n <= 0) return;
   hipLaunchKernelGGL(gram_combine_kernel, dim3(grid_of(n * n)), dim3(256), 0, ctx().stream, n,
                      C, ldc, M, ldm);
+  LAUNCH_CHECK();
+}
+
+i64 split_sym_gw_size(i64 n, i64 l, int max_split) { return (i64)max_split * n * 3 * l; }
+
+void split_sym_apply(const SplitSym &m, i64 l, const float *B, i64 ldb, float alpha, float *G,
+                     i64 ldg) {
+  const i64 n = m.n;
+  if (n <= 0 || l <= 0) return;
+  hipLaunchKernelGGL(split_sym_rhs_kernel, dim3(grid_of(n * l)), dim3(256), 0, ctx().stream, n,
+                     l, B, ldb, m.bop);
+  LAUNCH_CHECK();
+  // [Mh | Ml] = S^T (Mh, Ml symmetric), K = 2n
+  const int parts =
+      gemm_bf16_partials('T', 'N', n, 3 * l, 2 * n, m.S, 2 * n, m.bop, 2 * n, m.gw, m.max_split);
+  hipLaunchKernelGGL(split_sym_fold_kernel, dim3(grid_of(n * l)), dim3(256), 0, ctx().stream, n,
+                     l, parts, (const float *)m.gw, alpha, G, ldg);
+  LAUNCH_CHECK();
+}
+
+void split_sym_columns(const SplitSym &m, i64 l, float *B, i64 ldb) {
+  if (m.n <= 0 || l <= 0) return;
+  hipLaunchKernelGGL(split_sym_columns_kernel, dim3(grid_of(m.n * l)), dim3(256), 0,
+                     ctx().stream, m.n, l, m.S, B, ldb);
   LAUNCH_CHECK();
 }
 

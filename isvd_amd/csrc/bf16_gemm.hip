@@ -351,13 +351,18 @@ struct Operand {
 
 }  // namespace
 
-void gemm_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t *A, i64 lda,
-               const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc) {
-  if (m <= 0 || n <= 0) return;
+namespace {
+
+// Shared set-up of gemm_bf16 and gemm_bf16_partials: checks, operand copies where the kernel's
+// alignment rules need them, the split-K choice (at most max_split chunks) and the launch.
+// With W != nullptr the kernel writes raw partial sums there ([ksplit][n][m], also when ksplit
+// is 1) and nothing is reduced; otherwise C := alpha op(A) op(B) + beta C.  Returns ksplit.
+int run_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t *A, i64 lda,
+             const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc, float *W, int max_split) {
   ISVD_CHECK(k > 0, "gemm_bf16: k must be positive");
   const bool KA = (ta == 'T' || ta == 't');  // op(A) = A^T: K contiguous
   const bool KB = !(tb == 'T' || tb == 't');  // op(B) = B: K contiguous
-  ISVD_CHECK(lda >= (KA ? k : m) && ldb >= (KB ? k : n) && ldc >= m,
+  ISVD_CHECK(lda >= (KA ? k : m) && ldb >= (KB ? k : n) && (W || ldc >= m),
              "gemm_bf16: leading dimension too small (lda %lld ldb %lld ldc %lld)", lda, ldb, ldc);
   // stored extents: op(A) = A (m x k) or A^T (A is k x m); op(B) = B (k x n) or B^T (B n x k)
   Operand oa(A, lda, KA ? k : m, KA ? m : k, !KA);
@@ -382,9 +387,9 @@ void gemm_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_
     i64 want = kCUs / tiles;
     const i64 maxs = k / (16 * bg::BK);
     if (want > maxs) want = maxs;
-    if (want > 16) want = 16;
+    if (want > max_split) want = max_split;
     const size_t per = sizeof(float) * (size_t)(m * n);
-    while (want > 1 && per * (size_t)want > splitk_budget()) --want;
+    while (!W && want > 1 && per * (size_t)want > splitk_budget()) --want;
     ksplit = (int)(want < 1 ? 1 : want);
   }
   i64 kchunk = (k + ksplit - 1) / ksplit;
@@ -392,20 +397,44 @@ void gemm_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_
   ksplit = (int)((k + kchunk - 1) / kchunk);
   g.ksplit = ksplit;
   g.kchunk = kchunk;
-  if (ksplit > 1) g.W = static_cast<float *>(splitk_workspace(sizeof(float) * (size_t)(ksplit * m * n)));
+  if (W) {
+    // raw partials: chunk s at W + s m n (dense, ld m); a single chunk goes there unscaled
+    if (ksplit > 1) {
+      g.W = W;
+    } else {
+      g.C = W; g.ldc = m; g.alpha = 1.f; g.beta = 0.f;
+    }
+  } else if (ksplit > 1) {
+    g.W = static_cast<float *>(splitk_workspace(sizeof(float) * (size_t)(ksplit * m * n)));
+  }
   dim3 grid((unsigned)tiles, (unsigned)ksplit);
   if (KA && KB) hipLaunchKernelGGL((bf16_gemm_kernel<true, true>), grid, dim3(bg::NT), 0, ctx().stream, g);
   else if (KA) hipLaunchKernelGGL((bf16_gemm_kernel<true, false>), grid, dim3(bg::NT), 0, ctx().stream, g);
   else if (KB) hipLaunchKernelGGL((bf16_gemm_kernel<false, true>), grid, dim3(bg::NT), 0, ctx().stream, g);
   else hipLaunchKernelGGL((bf16_gemm_kernel<false, false>), grid, dim3(bg::NT), 0, ctx().stream, g);
   LAUNCH_CHECK();
-  if (ksplit > 1) {
+  if (!W && ksplit > 1) {
     i64 blocks = (m * n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
     hipLaunchKernelGGL(bf16_splitk_reduce_kernel, dim3((unsigned)blocks), dim3(256), 0,
                        ctx().stream, m, n, ksplit, (const float *)g.W, alpha, beta, C, ldc);
     LAUNCH_CHECK();
   }
+  return ksplit;
+}
+
+}  // namespace
+
+void gemm_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t *A, i64 lda,
+               const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc) {
+  if (m <= 0 || n <= 0) return;
+  (void)run_bf16(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 16);
+}
+
+int gemm_bf16_partials(char ta, char tb, i64 m, i64 n, i64 k, const uint16_t *A, i64 lda,
+                       const uint16_t *B, i64 ldb, float *W, int max_split) {
+  ISVD_CHECK(m > 0 && n > 0 && W && max_split >= 1, "gemm_bf16_partials: bad arguments");
+  return run_bf16(ta, tb, m, n, k, 1.f, A, lda, B, ldb, 0.f, nullptr, m, W, max_split);
 }
 
 }  // namespace isvd

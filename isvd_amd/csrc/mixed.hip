@@ -91,10 +91,22 @@ void stage_orth_tsqr_b(const Dims &d, float *yst, i64 ldyst) {
 void stage_integrate_kn_b(const Dims &d, const KnArgs &args, const float *yst, i64 ldyst,
                           float *qt, i64 ldqt, i64 *iter, double *err) {
   const i64 l = d.l, Nl = d.Nl;
-  DBuf<float> m32(Nl * Nl), x32(Nl * l);
-  gram_split(d, yst, ldyst, m32.get());
-  integrate_kn_coords<float>(d, args, m32.get(), x32.get(), iter, err);
-  m32.release();
+  DBuf<float> x32(Nl * l);
+  // M as its two BF16 terms: the FP32 loop iterates with M'' = Mh + Ml and takes its products
+  // with M'' on the BF16 matrix cores
+  DBuf<uint16_t> S(2 * Nl * Nl), bop(2 * Nl * 3 * l);
+  SplitSym ms;
+  ms.n = Nl;
+  ms.S = S.get();
+  ms.bop = bop.get();
+  DBuf<float> gw(split_sym_gw_size(Nl, l, ms.max_split));
+  ms.gw = gw.get();
+  {
+    DBuf<float> m32(Nl * Nl);
+    gram_split(d, yst, ldyst, m32.get());
+    split_bf16(Nl, Nl, 2, m32.get(), Nl, S.get(), 2 * Nl);
+  }
+  integrate_kn_coords<float>(d, args, nullptr, x32.get(), iter, err, &ms);
   gemm<float>('T', 'N', l, d.m, Nl, 1.f, x32.get(), Nl, yst, ldyst, 0.f, qt, ldqt);
 }
 

@@ -3,8 +3,10 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 
 #include "la.hpp"
+#include "mx.hpp"
 
 namespace isvd {
 
@@ -350,8 +352,21 @@ int kn_mode() {
 // ring) with C and C^-1 from the eigendecomposition of Z.
 template <typename T>
 void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out, i64 *iter_out,
-                         double *err_out) {
+                         double *err_out, const SplitSym *ms) {
   const i64 l = d.l, Nl = d.Nl, ll = l * l;
+  ISVD_CHECK(!ms || (std::is_same_v<T, float> && ms->n == Nl),
+             "KN: the BF16-split M is for the FP32 loop with n = Nl");
+  // out := alpha M in (Nl x l, ld Nl)
+  auto m_apply = [&](const T *in, T alpha, T *out) {
+    if constexpr (std::is_same_v<T, float>) {
+      if (ms) {
+        split_sym_apply(*ms, l, in, Nl, alpha, out, Nl);
+        return;
+      }
+    }
+    // (in^T M)^T: M is exactly symmetric
+    gemm_ct<T>('T', 'N', l, Nl, Nl, alpha, in, Nl, M, Nl, T(0), out, Nl);
+  };
   const T ninv = T(1) / T(d.N);
   const int mode = kn_mode();
   // [D | Dg] as K-chunk partial sums over about 256 workgroups of 64 x 64 tiles
@@ -384,7 +399,7 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
     HIP_CHECK(hipStreamWaitEvent(ctx().aux, ctx().fork, 0));
     {
       OnStream on(ctx().aux);
-      gemm_ct<T>('T', 'N', l, Nl, Nl, ninv, Bg(q), Nl, M, Nl, T(0), G(q), Nl);
+      m_apply(Bg(q), ninv, G(q));
     }
     HIP_CHECK(hipEventRecord(ctx().join, ctx().aux));
   };
@@ -411,10 +426,15 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
     updates(q);
   };
   // X_c := [I; 0] (Qc := Q_0, :156);  B_c := Qs^T Q_0 = M[:, 0:l]  (:159-160);
-  // Bg_c := M B_c / N (:174-178), as (B^T M)^T: M is exactly symmetric
+  // Bg_c := M B_c / N (:174-178)
   fill<T>(Nl, l, T(0), X(0), Nl, T(1));
-  omatcopy<T>('N', Nl, l, T(1), M, Nl, B(0), Nl);
-  gemm_ct<T>('T', 'N', l, Nl, Nl, ninv, B(0), Nl, M, Nl, T(0), Bg(0), Nl);
+  if constexpr (std::is_same_v<T, float>) {
+    if (ms) split_sym_columns(*ms, l, B(0), Nl);
+    else omatcopy<T>('N', Nl, l, T(1), M, Nl, B(0), Nl);
+  } else {
+    omatcopy<T>('N', Nl, l, T(1), M, Nl, B(0), Nl);
+  }
+  m_apply(B(0), ninv, Bg(0));
   HIP_CHECK(hipEventRecord(ctx().fork, ctx().stream));
   gram(0);
   enqueue(1, mode);
@@ -764,7 +784,7 @@ void stage_post_symmetric(const Dims &d, char ordera, const T *a, i64 lda, const
   template void stage_post_tsqr<T>(const Dims &, char, const T *, i64, const T *, i64, T *,    \
                                    T *, i64, T *, i64, bool, bool);                            \
   template void integrate_kn_coords<T>(const Dims &, const KnArgs &, const T *, T *, i64 *,    \
-                                       double *);                                              \
+                                       double *, const SplitSym *);                            \
   template void integrate_wy_coords<T>(const Dims &, const WyArgs &, const T *, T *, i64 *,    \
                                        double *);                                              \
   template void post_gramian_z<T>(const Dims &, const T *, i64, T *, T *, T *, i64, T *, i64,  \

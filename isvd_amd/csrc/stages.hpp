@@ -74,10 +74,13 @@ void stage_post_symmetric(const Dims &d, char ordera, const T *a, i64 lda, const
                           T *s, T *ut, i64 ldut, bool want_u);
 
 // The integration loops in the coordinates of the sketch basis, from M = Qs^T Qs (Nl x Nl):
-// x_out (Nl x l) receives the coefficients of Qbar = Qs x_out.
+// x_out (Nl x l) receives the coefficients of Qbar = Qs x_out.  The FP32 KN loop may take M as
+// its two BF16 terms instead (ms != nullptr, M unused): it then iterates with M'' = Mh + Ml and
+// forms its products with M on the BF16 matrix cores (split_sym_apply).
+struct SplitSym;
 template <typename T>
 void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out, i64 *iter,
-                         double *err);
+                         double *err, const SplitSym *ms = nullptr);
 
 template <typename T>
 void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out, i64 *iter,

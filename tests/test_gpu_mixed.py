@@ -120,6 +120,29 @@ def test_integrate_mixed_golden(golden, torch_cuda, alg, it_ref):
     assert np.abs(qh @ qh.T - ref @ ref.T).max() < 1e-4
 
 
+def test_integrate_mixed_kn_at_the_workload_sketch_count(torch_cuda):
+    """The FP32 KN loop at N = 32, l = 128 (Nl = 4096, the 131072 x 16384 workload's sketch
+    count and size): M held as its two BF16 terms, every product with M on the BF16 matrix
+    cores (five terms against the exact three-term split of the right-hand side, K-chunk
+    partials folded in a fixed order).  Against the FP64 stage on the same FP64-orthonormal
+    bases: the same iteration count (+-1) and Qbar's subspace within the mixed path's 1e-4
+    target."""
+    torch = torch_cuda
+    m, n, l, N = 8192, 4096, 128, 32
+    rng = np.random.default_rng(21)
+    a = lowrank_matrix(m, n, 1.0 / np.arange(1, 2049), seed=12, noise=1e-6)
+    y = a @ rng.standard_normal((n, N * l))
+    qs = np.concatenate([np.linalg.qr(y[:, i * l:(i + 1) * l])[0] for i in range(N)], axis=1)
+    p = capi.create_param(m, n, 100, l - 100, N)
+    q64, it64, _ = pipeline.integrate(p, dev(torch, qs, np.float64), "KN", [256, 1e-4])
+    q32, it32, _ = pipeline.integrate(p, dev(torch, qs, np.float32), "KN", [256, 1e-4],
+                                      mixed=True)
+    ang = principal_angles(q32.double().cpu().numpy(), q64.cpu().numpy()).max()
+    print(f"mixed KN N=32: iter {it32} (FP64 {it64}), angle {ang:.2e}")
+    assert it64 < 256 and abs(it32 - it64) <= 1
+    assert ang < 1e-4
+
+
 @pytest.mark.parametrize("alg", ["GR", "TS"])
 def test_postprocess_mixed_golden(golden, torch_cuda, alg):
     """sigma of Qbar^T A with A rounded to BF16 against the FP64 postprocess of the same
