@@ -23,42 +23,48 @@ template <> struct Eps<double> { static constexpr double v = DBL_EPSILON; };
 template <> struct Eps<float> { static constexpr float v = FLT_EPSILON; };
 
 constexpr int kMaxSweeps = 40;
-// 16 waves, 16 lanes per column pair at n = 128 (8 rows per lane): a round is a chain of LDS
-// reads, three reductions and the scalar rotation formula, and 16 waves hide it best (4 and 8
-// waves per matrix were measured slower).
-constexpr int kJacobiThreads = 1024;
-constexpr int kMaxRpl = 8;  // rows per lane: n <= 128 rows over >= 16 lanes per pair
+// 16 lanes per column pair, each holding every 16th row of the pair's two columns: 4 pairs per
+// wave and, at n = 128, 16 waves (4 per SIMD), which hide each other's LDS, DPP, division and
+// (with V) global-memory latencies.  A round is bound by instruction issue and latency together:
+// 8 lanes per pair (8 waves) and 4 lanes (4 waves) were measured slower at n = 128.
+constexpr int kTpp = 16;
+constexpr int kMaxThreads = 1024;
 
-// round-robin (circle method) pairing of np players (np even), round r in [0, np-1)
+// rows per lane for an n x n problem with tpp lanes per pair (a power of two, n <= tpp rows)
+inline int jacobi_rpl(i64 n, int tpp) {
+  int r = 1;
+  while (r * tpp < n) r *= 2;
+  return r;
+}
+
+// round-robin (circle method) pairing of np players (np even), round r in [0, np-1), table
+// k in [0, np/2): player m = np - 1 stays at table 0, the others turn (r, k < m: no modulo)
 __device__ __forceinline__ void rr_pair(int np, int r, int k, int &p, int &q) {
   const int m = np - 1;
   if (k == 0) {
-    p = r % m;
+    p = r;
     q = m;
   } else {
-    p = (r + k) % m;
-    q = (r - k + m) % m;
+    p = (r + k >= m) ? r + k - m : r + k;
+    q = (r >= k) ? r - k : r - k + m;
   }
-}
-
-// sum over aligned groups of `width` lanes (a power of two <= 64)
-template <typename T>
-__device__ __forceinline__ T group_sum(T v, int width) {
-  return lane_group_sum(v, width);
 }
 
 // mode 0: SVD of A (general n x n).  mode 1: symmetric eigendecomposition of the matrix whose
 // upper triangle is stored in A, computed as the SVD of sym(A) + c I with c = 2 ||sym(A)||_F.
-template <typename T, bool WANT_V>
-__global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
+// TPP lanes per pair, RPL rows per lane; the columns are padded with zero rows to TPP RPL
+// (rotations keep them zero), so the round loop has no row bounds.  Vwork holds V with the same
+// padded rows (TPP RPL x n).  blockDim.x = TPP lanes per pair, rounded up to whole waves.
+template <typename T, bool WANT_V, int TPP, int RPL>
+__global__ __launch_bounds__(kMaxThreads) void svd_jacobi_kernel(
     int n, int mode, const T *__restrict__ A, i64 lda, i64 sA, T *__restrict__ sig, i64 lds,
     T *__restrict__ U, i64 ldu, i64 sU, T *__restrict__ Vwork, T *__restrict__ V, i64 ldv,
     i64 sV, T *__restrict__ shift_out) {
   extern __shared__ unsigned char smem_raw[];
-  // n x n, leading dimension n + 1: the four column pairs a wave works on start in
+  // n x n, leading dimension n + 1: the columns of the pairs a wave works on start in
   // different banks
   T *G = reinterpret_cast<T *>(smem_raw);
-  __shared__ T red[kJacobiThreads / 64];
+  __shared__ T red[kMaxThreads / 64];
   __shared__ T nrm[128];
   __shared__ int rank_of[128];
   __shared__ int n_rot;
@@ -66,53 +72,53 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
 
   const int b = blockIdx.x;
   const int tid = threadIdx.x;
+  const int nt = blockDim.x, waves = nt / 64;
   const T *Ab = A + (i64)b * sA;
   const int np = (n + 1) & ~1;
   const int npairs = np / 2;
-  // lanes per pair: power of two, <= 64
-  int tpp = kJacobiThreads / npairs;
-  int t2 = 1;
-  while (t2 * 2 <= tpp && t2 < 64) t2 *= 2;
-  tpp = t2;
-  const int grp = tid / tpp, sub = tid % tpp;
+  const int grp = tid / TPP, sub = tid % TPP;
+  constexpr int R = TPP * RPL;  // padded rows
+  constexpr int ld = R + 1;
 
-  // ---- load ----
-  for (int e = tid; e < n * n; e += kJacobiThreads) {
-    const int i = e % n, j = e / n;
-    T v;
-    if (mode == 1) {
-      v = (i <= j) ? Ab[i + (i64)j * lda] : Ab[j + (i64)i * lda];  // symmetric from upper
-    } else {
-      v = Ab[i + (i64)j * lda];
+  // ---- load (rows n .. R-1 zero) ----
+  for (int e = tid; e < R * n; e += nt) {
+    const int i = e % R, j = e / R;
+    T v = T(0);
+    if (i < n) {
+      if (mode == 1) {
+        v = (i <= j) ? Ab[i + (i64)j * lda] : Ab[j + (i64)i * lda];  // symmetric from upper
+      } else {
+        v = Ab[i + (i64)j * lda];
+      }
     }
-    G[i + j * (n + 1)] = v;
+    G[i + j * ld] = v;
   }
   __syncthreads();
   if (mode == 1) {
     // c = 2 ||A||_F (+1 when A == 0), deterministic block reduction
     T part = 0;
-    for (int e = tid; e < n * n; e += kJacobiThreads) {
-      const T g = G[e % n + (e / n) * (n + 1)];
+    for (int e = tid; e < n * n; e += nt) {
+      const T g = G[e % n + (e / n) * ld];
       part += g * g;
     }
-    part = group_sum(part, 64);
+    part = lane_group_sum(part, 64);
     if ((tid & 63) == 0) red[tid >> 6] = part;
     __syncthreads();
     if (tid == 0) {
       T s = 0;
-      for (int w = 0; w < kJacobiThreads / 64; ++w) s += red[w];
+      for (int w = 0; w < waves; ++w) s += red[w];
       T c = T(2) * sqrt(s);
       if (c == T(0)) c = T(1);
       shift_sh = c;
     }
     __syncthreads();
-    for (int i = tid; i < n; i += kJacobiThreads) G[i + i * (n + 1)] += shift_sh;
+    for (int i = tid; i < n; i += nt) G[i + i * ld] += shift_sh;
     __syncthreads();
   }
-  T *Vb = WANT_V ? (Vwork + (i64)b * n * n) : nullptr;
+  T *Vb = WANT_V ? (Vwork + (i64)b * R * n) : nullptr;
   if (WANT_V) {
-    for (int e = tid; e < n * n; e += kJacobiThreads) {
-      const int i = e % n, j = e / n;
+    for (int e = tid; e < R * n; e += nt) {
+      const int i = e % R, j = e / R;
       Vb[e] = (i == j) ? T(1) : T(0);
     }
     __syncthreads();
@@ -123,48 +129,54 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
     if (tid == 0) n_rot = 0;
     __syncthreads();
     for (int r = 0; r < np - 1; ++r) {
-      if (grp < npairs) {
-        int p, q;
-        rr_pair(np, r, grp, p, q);
-        if (p < n && q < n) {
-          // the two columns stay in registers between the dot products and the rotation
-          T gp[kMaxRpl], gq[kMaxRpl];
-          T alpha = 0, beta = 0, gamma = 0;
+      int p = n, q = n;
+      if (grp < npairs) rr_pair(np, r, grp, p, q);
+      if (p < n && q < n) {
+        // the V columns are requested first: their latency runs under the dot products
+        T vp[RPL], vq[RPL];
+        const T *vcp = Vb + (i64)p * R + sub, *vcq = Vb + (i64)q * R + sub;
+        if (WANT_V) {
 #pragma unroll
-          for (int u = 0; u < kMaxRpl; ++u) {
-            const int i = sub + u * tpp;
-            gp[u] = (i < n) ? G[i + p * (n + 1)] : T(0);
-            gq[u] = (i < n) ? G[i + q * (n + 1)] : T(0);
-            alpha += gp[u] * gp[u];
-            beta += gq[u] * gq[u];
-            gamma += gp[u] * gq[u];
+          for (int u = 0; u < RPL; ++u) {
+            vp[u] = vcp[u * TPP];
+            vq[u] = vcq[u * TPP];
           }
-          alpha = group_sum(alpha, tpp);
-          beta = group_sum(beta, tpp);
-          gamma = group_sum(gamma, tpp);
-          // |gamma| > tol sqrt(alpha beta), compared in squares
-          if (gamma != T(0) && gamma * gamma > tol * tol * (alpha * beta)) {
-            const T zeta = (beta - alpha) / (T(2) * gamma);
-            const T t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
-            const T c = T(1) / sqrt(T(1) + t * t);
-            const T s = c * t;
+        }
+        // the two columns stay in registers between the dot products and the rotation
+        T *cp = G + p * ld + sub, *cq = G + q * ld + sub;
+        T gp[RPL], gq[RPL];
+        T alpha = 0, beta = 0, gamma = 0;
 #pragma unroll
-            for (int u = 0; u < kMaxRpl; ++u) {
-              const int i = sub + u * tpp;
-              if (i < n) {
-                G[i + p * (n + 1)] = c * gp[u] - s * gq[u];
-                G[i + q * (n + 1)] = s * gp[u] + c * gq[u];
-              }
-            }
-            if (WANT_V) {
-              for (int i = sub; i < n; i += tpp) {
-                const T vp = Vb[i + (i64)p * n], vq = Vb[i + (i64)q * n];
-                Vb[i + (i64)p * n] = c * vp - s * vq;
-                Vb[i + (i64)q * n] = s * vp + c * vq;
-              }
-            }
-            if (sub == 0) atomicAdd(&n_rot, 1);
+        for (int u = 0; u < RPL; ++u) {
+          gp[u] = cp[u * TPP];
+          gq[u] = cq[u * TPP];
+          alpha += gp[u] * gp[u];
+          beta += gq[u] * gq[u];
+          gamma += gp[u] * gq[u];
+        }
+        alpha = lane_group_sum(alpha, TPP);
+        beta = lane_group_sum(beta, TPP);
+        gamma = lane_group_sum(gamma, TPP);
+        // |gamma| > tol sqrt(alpha beta), compared in squares
+        if (gamma != T(0) && gamma * gamma > tol * tol * (alpha * beta)) {
+          const T zeta = (beta - alpha) / (T(2) * gamma);
+          const T t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
+          const T c = T(1) / sqrt(T(1) + t * t);
+          const T s = c * t;
+#pragma unroll
+          for (int u = 0; u < RPL; ++u) {
+            cp[u * TPP] = c * gp[u] - s * gq[u];
+            cq[u * TPP] = s * gp[u] + c * gq[u];
           }
+          if (WANT_V) {
+            T *wp = Vb + (i64)p * R + sub, *wq = Vb + (i64)q * R + sub;
+#pragma unroll
+            for (int u = 0; u < RPL; ++u) {
+              wp[u * TPP] = c * vp[u] - s * vq[u];
+              wq[u * TPP] = s * vp[u] + c * vq[u];
+            }
+          }
+          if (sub == 0) atomicAdd(&n_rot, 1);
         }
       }
       __syncthreads();
@@ -174,14 +186,14 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
   }
 
   // ---- singular values (column norms), ranks for descending order ----
-  for (int j = tid / 64; j < n; j += kJacobiThreads / 64) {
+  for (int j = tid / 64; j < n; j += waves) {
     T a = 0;
-    for (int i = tid % 64; i < n; i += 64) a += G[i + j * (n + 1)] * G[i + j * (n + 1)];
-    a = group_sum(a, 64);
+    for (int i = tid % 64; i < n; i += 64) a += G[i + j * ld] * G[i + j * ld];
+    a = lane_group_sum(a, 64);
     if ((tid & 63) == 0) nrm[j] = sqrt(a);
   }
   __syncthreads();
-  for (int j = tid; j < n; j += kJacobiThreads) {
+  for (int j = tid; j < n; j += nt) {
     int rk = 0;
     const T sj = nrm[j];
     for (int i = 0; i < n; ++i) {
@@ -191,22 +203,22 @@ __global__ __launch_bounds__(kJacobiThreads) void svd_jacobi_kernel(
     rank_of[j] = rk;
   }
   __syncthreads();
-  for (int j = tid; j < n; j += kJacobiThreads) sig[(i64)b * lds + rank_of[j]] = nrm[j];
+  for (int j = tid; j < n; j += nt) sig[(i64)b * lds + rank_of[j]] = nrm[j];
   if (mode == 1 && tid == 0 && shift_out) shift_out[b] = shift_sh;
   if (U) {
     T *Ub = U + (i64)b * sU;
-    for (int e = tid; e < n * n; e += kJacobiThreads) {
+    for (int e = tid; e < n * n; e += nt) {
       const int i = e % n, j = e / n;
       const T sj = nrm[j];
-      Ub[i + (i64)rank_of[j] * ldu] = (sj > T(0)) ? G[i + j * (n + 1)] / sj : T(0);
+      Ub[i + (i64)rank_of[j] * ldu] = (sj > T(0)) ? G[i + j * ld] / sj : T(0);
     }
   }
   if (WANT_V && V) {
     __syncthreads();
     T *Vo = V + (i64)b * sV;
-    for (int e = tid; e < n * n; e += kJacobiThreads) {
+    for (int e = tid; e < n * n; e += nt) {
       const int i = e % n, j = e / n;
-      Vo[i + (i64)rank_of[j] * ldv] = Vb[e];
+      Vo[i + (i64)rank_of[j] * ldv] = Vb[i + (i64)j * R];
     }
   }
 }
@@ -223,29 +235,50 @@ __global__ void syev_finish_kernel(int n, const T *sig, const T *shift, const T 
   }
 }
 
+template <typename T, bool WANT_V, int TPP, int RPL>
+void launch_rpl(int nt, i64 n, int mode, const T *A, i64 lda, i64 sA, T *s, i64 lds, T *U,
+                i64 ldu, i64 sU, T *vwork, T *V, i64 ldv, i64 sV, i64 batch, T *shift_out) {
+  const size_t shmem = sizeof(T) * (size_t)(TPP * RPL + 1) * (size_t)n;
+  auto kern = svd_jacobi_kernel<T, WANT_V, TPP, RPL>;
+  HIP_CHECK(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)shmem));
+  hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(nt), shmem, ctx().stream, (int)n, mode, A,
+                     lda, sA, s, lds, U, ldu, sU, vwork, V, ldv, sV, shift_out);
+  LAUNCH_CHECK();
+}
+
+template <typename T, bool WANT_V>
+void launch_v(i64 n, int mode, const T *A, i64 lda, i64 sA, T *s, i64 lds, T *U, i64 ldu, i64 sU,
+              T *vwork, T *V, i64 ldv, i64 sV, i64 batch, T *shift_out) {
+  constexpr int TPP = kTpp;
+  const i64 npairs = (n + 1) / 2;
+  const int nt = (int)((npairs * TPP + 63) / 64 * 64);
+#define ISVD_JAC(R) \
+  launch_rpl<T, WANT_V, TPP, R>(nt, n, mode, A, lda, sA, s, lds, U, ldu, sU, vwork, V, ldv, sV,   \
+                                batch, shift_out)
+  switch (jacobi_rpl(n, TPP)) {
+    case 1: ISVD_JAC(1); break;
+    case 2: ISVD_JAC(2); break;
+    case 4: ISVD_JAC(4); break;
+    case 8: ISVD_JAC(8); break;
+    default: ISVD_JAC(128 / TPP); break;
+  }
+#undef ISVD_JAC
+}
+
 template <typename T>
 void launch_svd(i64 n, int mode, const T *A, i64 lda, i64 sA, T *s, i64 lds, T *U, i64 ldu,
                 i64 sU, T *V, i64 ldv, i64 sV, i64 batch, T *shift_out) {
   ISVD_CHECK(n >= 1 && n <= 128, "small SVD supports 1 <= n <= 128 (got %lld)", n);
-  const size_t shmem = sizeof(T) * (size_t)n * (size_t)(n + 1);
-  DBuf<T> vwork;
   if (V) {
-    vwork.alloc(batch * n * n);
-    auto kern = svd_jacobi_kernel<T, true>;
-    HIP_CHECK(hipFuncSetAttribute((const void *)kern,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(kJacobiThreads), shmem, ctx().stream,
-                       (int)n, mode, A, lda, sA, s, lds, U, ldu, sU, vwork.get(), V, ldv, sV,
-                       shift_out);
+    // V with its rows padded like the columns of the matrix (at most 128 rows)
+    DBuf<T> vwork(batch * (i64)kTpp * jacobi_rpl(n, kTpp) * n);
+    launch_v<T, true>(n, mode, A, lda, sA, s, lds, U, ldu, sU, vwork.get(), V, ldv, sV, batch,
+                      shift_out);
   } else {
-    auto kern = svd_jacobi_kernel<T, false>;
-    HIP_CHECK(hipFuncSetAttribute((const void *)kern,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
-    hipLaunchKernelGGL(kern, dim3((unsigned)batch), dim3(kJacobiThreads), shmem, ctx().stream,
-                       (int)n, mode, A, lda, sA, s, lds, U, ldu, sU, (T *)nullptr, (T *)nullptr,
-                       (i64)0, (i64)0, shift_out);
+    launch_v<T, false>(n, mode, A, lda, sA, s, lds, U, ldu, sU, (T *)nullptr, (T *)nullptr,
+                       (i64)0, (i64)0, batch, shift_out);
   }
-  LAUNCH_CHECK();
 }
 
 }  // namespace
