@@ -49,13 +49,13 @@ __global__ void sum_blocks_kernel(i64 rows, i64 cols, int parts, const float *sr
   }
 }
 
-// M(i, j) := (HH(i, j) + HH(j, i)) / 2 + (LH(i, j) + LH(j, i)) with HH = rows 0..n-1 and
-// LH = rows n..2n-1 of C (2n x n): hi*hi + hi*lo + lo*hi, exactly symmetric
+// M(i, j) := HH(min(i, j), max(i, j)) + (LH(i, j) + LH(j, i)) with HH = rows 0..n-1 (its
+// upper triangle) and LH = rows n..2n-1 of C (2n x n): hi*hi + hi*lo + lo*hi, exactly symmetric
 __global__ void gram_combine_kernel(i64 n, const float *C, i64 ldc, float *M, i64 ldm) {
   for (i64 e = blockIdx.x * (i64)blockDim.x + threadIdx.x; e < n * n;
        e += (i64)gridDim.x * blockDim.x) {
     const i64 i = e % n, j = e / n;
-    const float hh = 0.5f * (C[i + j * ldc] + C[j + i * ldc]);
+    const float hh = (i <= j) ? C[i + j * ldc] : C[j + i * ldc];
     const float lh = C[n + i + j * ldc] + C[n + j + i * ldc];
     M[i + j * ldm] = hh + lh;
   }

@@ -63,6 +63,7 @@ struct BgArgs {
   int ksplit;       // K chunks (grid.y)
   i64 kchunk;       // K values per chunk (multiple of BK)
   float *W;         // split-K partials: [ksplit][N][M]
+  i64 upper_rows;   // tiles inside rows 0 .. upper_rows-1 strictly below the diagonal are skipped
 };
 
 namespace bg {
@@ -101,6 +102,9 @@ __global__ __launch_bounds__(bg::NT) void bf16_gemm_kernel(BgArgs g) {
   const i64 gsz = (tiles_n - first_n) < kGroup ? (tiles_n - first_n) : kGroup;
   const i64 within = pid % (kGroup * tiles_m);
   const i64 m0 = (within / gsz) * BM, n0 = (first_n + within % gsz) * BN;
+  // the caller mirrors C's symmetric top block (BM == BN: a tile below the diagonal holds only
+  // elements below it)
+  if (m0 > n0 && m0 + BM <= g.upper_rows) return;
   const int ks = blockIdx.y;
   const i64 kbeg = (i64)ks * g.kchunk;
   const i64 kend = (kbeg + g.kchunk < g.K) ? kbeg + g.kchunk : g.K;
@@ -358,7 +362,8 @@ namespace {
 // With W != nullptr the kernel writes raw partial sums there ([ksplit][n][m], also when ksplit
 // is 1) and nothing is reduced; otherwise C := alpha op(A) op(B) + beta C.  Returns ksplit.
 int run_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t *A, i64 lda,
-             const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc, float *W, int max_split) {
+             const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc, float *W, int max_split,
+             i64 upper_rows = 0) {
   ISVD_CHECK(k > 0, "gemm_bf16: k must be positive");
   const bool KA = (ta == 'T' || ta == 't');  // op(A) = A^T: K contiguous
   const bool KB = !(tb == 'T' || tb == 't');  // op(B) = B: K contiguous
@@ -373,6 +378,7 @@ int run_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t 
   g.nb = KB ? n : ob.rows_fetch;
   g.A = oa.p; g.lda = oa.ld; g.B = ob.p; g.ldb = ob.ld;
   g.C = C; g.ldc = ldc; g.alpha = alpha; g.beta = beta;
+  g.upper_rows = upper_rows;
   // per-lane fill offsets are 32-bit
   ISVD_CHECK(256 * g.lda < (1ll << 32) && 256 * g.ldb < (1ll << 32),
              "gemm_bf16: leading dimensions too large (%lld, %lld)", g.lda, g.ldb);
@@ -429,6 +435,12 @@ void gemm_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_
                const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc) {
   if (m <= 0 || n <= 0) return;
   (void)run_bf16(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, nullptr, 16);
+}
+
+void gemm_bf16_upper(char ta, char tb, i64 m, i64 n, i64 k, i64 upper_rows, const uint16_t *A,
+                     i64 lda, const uint16_t *B, i64 ldb, float *C, i64 ldc) {
+  if (m <= 0 || n <= 0) return;
+  (void)run_bf16(ta, tb, m, n, k, 1.f, A, lda, B, ldb, 0.f, C, ldc, nullptr, 16, upper_rows);
 }
 
 int gemm_bf16_partials(char ta, char tb, i64 m, i64 n, i64 k, const uint16_t *A, i64 lda,

@@ -32,8 +32,9 @@ void gram_split(const Dims &d, const float *yst, i64 ldyst, float *m32) {
   DBuf<uint16_t> S(2 * Nl * m);
   split_bf16(Nl, m, 2, yst, ldyst, S.get(), 2 * Nl);
   DBuf<float> C(2 * Nl * Nl);
-  gemm_bf16('N', 'T', 2 * Nl, Nl, m, 1.f, S.get(), 2 * Nl, S.get(), 2 * Nl, 0.f, C.get(),
-                2 * Nl);
+  // H^T H is symmetric: its tiles below the diagonal are skipped (gram_combine reads its upper
+  // triangle)
+  gemm_bf16_upper('N', 'T', 2 * Nl, Nl, m, Nl, S.get(), 2 * Nl, S.get(), 2 * Nl, C.get(), 2 * Nl);
   S.release();
   gram_combine(Nl, C.get(), 2 * Nl, m32, Nl);
 }

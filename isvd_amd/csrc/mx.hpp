@@ -14,6 +14,13 @@ namespace isvd {
 void gemm_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t *A, i64 lda,
                const uint16_t *B, i64 ldb, float beta, float *C, i64 ldc);
 
+// C := op(A) op(B) as gemm_bf16 (alpha 1, beta 0), except that the elements of C's top
+// upper_rows x upper_rows block below its diagonal are not all written: the tiles of that block
+// strictly below the diagonal are skipped.  For a symmetric top block, read it as
+// C(min(i, j), max(i, j)).
+void gemm_bf16_upper(char ta, char tb, i64 m, i64 n, i64 k, i64 upper_rows, const uint16_t *A,
+                     i64 lda, const uint16_t *B, i64 ldb, float *C, i64 ldc);
+
 // Raw split-K partial sums of op(A) op(B) for BF16 A, B: W[s] (m x n, dense, ld m) is the
 // product over the s-th of at most max_split K chunks (FP32 accumulation).  Returns the number of
 // chunks written; the caller sums them.
@@ -56,7 +63,8 @@ void split_bf16(i64 rows, i64 cols, int parts, const float *src, i64 lds, uint16
 // dst (rows x cols, ldd) := the sum of the `parts` row blocks of src (parts*rows x cols)
 void sum_blocks(i64 rows, i64 cols, int parts, const float *src, i64 lds, float *dst, i64 ldd);
 
-// M (n x n) := sym(HH) + LH + LH^T from C = [H^T H ; L^T H] (2n x n)
+// M (n x n) := HH + LH + LH^T from C = [H^T H ; L^T H] (2n x n), HH read from its upper
+// triangle (HH(min(i, j), max(i, j))), so M is exactly symmetric
 void gram_combine(i64 n, const float *C, i64 ldc, float *M, i64 ldm);
 
 }  // namespace isvd
