@@ -75,6 +75,18 @@ void syev_small(i64 n, const T *A, i64 lda, T *w, T *Z, i64 ldz);
 template <typename T>
 void tsqr_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr, i64 sR);
 
+// The same factorization (Y overwritten by Q, R = the l x l factor when R != null) by
+// CholeskyQR2 on the matrix cores: two passes of Gram, Cholesky and Y R^-1.  Returns false,
+// with Y and R untouched, unless the device certifies that the result is as accurate as
+// Householder QR (every pivot positive and ||Q1^T Q1 - I||_F <= 1/2 after the first pass); R's
+// diagonal is positive.
+template <typename T>
+bool cholqr2_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr, i64 sR);
+// Which tall-skinny QR the stages take (ISVD_TS): 0 (unset) CholeskyQR2 where the caller allows
+// it and the device certifies it, else the Householder tree; 1 ("householder") the Householder
+// tree only; 2 ("cholqr") CholeskyQR2 wherever certified (test hook for the stage entry points).
+int tsqr_mode();
+
 // l x l products (l <= 128): C := alpha op(A) op(B) + beta C + diag I; one short launch.
 template <typename T>
 void small_gemm(char ta, char tb, i64 m, i64 n, i64 k, T alpha, const T *A, i64 lda, const T *B,

@@ -83,8 +83,8 @@ void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst) {
 }
 
 // @x@_orthogonalize_tall_skinny_qr.c:127-143 in FP32
-void stage_orth_tsqr_b(const Dims &d, float *yst, i64 ldyst) {
-  stage_orth_tsqr<float>(d, yst, ldyst);
+void stage_orth_tsqr_b(const Dims &d, float *yst, i64 ldyst, bool cholqr) {
+  stage_orth_tsqr<float>(d, yst, ldyst, cholqr);
 }
 
 // @x@_integrate_kolmogorov_nagumo.c:155-253.  The iteration runs in FP32 on the FP32-accurate

@@ -18,8 +18,9 @@ void stage_sketch_b(const Dims &d, char ordera, const uint16_t *a, i64 lda,
 // MFMA; the l x l eigendecompositions in FP64)
 void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst);
 
-// FP32 Householder TSQR
-void stage_orth_tsqr_b(const Dims &d, float *yst, i64 ldyst);
+// FP32 tall-skinny QR (stage_orth_tsqr: CholeskyQR2 when cholqr and certified, else the
+// Householder tree)
+void stage_orth_tsqr_b(const Dims &d, float *yst, i64 ldyst, bool cholqr = false);
 
 // M = Qs^T Qs by split-BF16 x3 products (FP32 accuracy), iterations in FP64 coordinates,
 // Qbar^T := X^T Qs^T in FP32

@@ -281,9 +281,16 @@ void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst) {
 }
 
 // Tall-skinny QR orthogonalization (@x@_orthogonalize_tall_skinny_qr.c:127-200): every sketch
-// Y_i (row-major m x l inside yst) is replaced by the orthonormal factor of its QR.
+// Y_i (row-major m x l inside yst) is replaced by the orthonormal factor of its QR.  With
+// cholqr, by CholeskyQR2 on the matrix cores when the device certifies it; else (and for
+// Wen-Yin, whose stopping point within its tolerance moves with rounding-level changes of the
+// bases) by the Householder tree.
 template <typename T>
-void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst) {
+void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst, bool cholqr) {
+  const int mode = tsqr_mode();
+  if (((cholqr && mode == 0) || mode == 2) &&
+      cholqr2_rowmajor<T>(d.m, d.l, yst, ldyst, d.l, d.N, nullptr, 0, 0))
+    return;
   tsqr_rowmajor<T>(d.m, d.l, yst, ldyst, d.l, d.N, nullptr, 0, 0);
 }
 
@@ -726,7 +733,8 @@ void post_tsqr_z(const Dims &d, const T *qt, i64 ldqt, T *zt, T *s, T *ut, i64 l
   ISVD_CHECK(d.n >= l, "tall-skinny QR postprocessing needs n >= l (n = %lld, l = %lld)", d.n, l);
   DBuf<T> r(l * l), lmat(l * l), ul(l * l), vl(l * l);
   // Z (row-major n x l in zt) = Q_z R  <=>  Z^T = R^T Q_z^T = L Q_lq   (:199-208)
-  tsqr_rowmajor<T>(d.n, l, zt, l, 0, 1, r.get(), l, 0);
+  if (tsqr_mode() == 1 || !cholqr2_rowmajor<T>(d.n, l, zt, l, 0, 1, r.get(), l, 0))
+    tsqr_rowmajor<T>(d.n, l, zt, l, 0, 1, r.get(), l, 0);
   hipLaunchKernelGGL(transpose_small_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
                      ctx().stream, (int)l, (const T *)r.get(), l, lmat.get(), l);
   LAUNCH_CHECK();
@@ -774,7 +782,7 @@ void stage_post_symmetric(const Dims &d, char ordera, const T *a, i64 lda, const
   template void project_host_blocks<T>(const Dims &, char, const T *, i64, const T *, i64, T *,  \
                                        i64);                                                   \
   template void stage_orth_gramian<T>(const Dims &, T *, i64);                                \
-  template void stage_orth_tsqr<T>(const Dims &, T *, i64);                                   \
+  template void stage_orth_tsqr<T>(const Dims &, T *, i64, bool);                             \
   template void stage_integrate_kn<T>(const Dims &, const KnArgs &, const T *, i64, T *, i64,  \
                                       i64 *, double *);                                        \
   template void stage_integrate_wy<T>(const Dims &, const WyArgs &, const T *, i64, T *, i64,  \

@@ -43,8 +43,12 @@ void project_host_blocks(const Dims &d, char ordera, const T *a_host, i64 lda, c
 template <typename T>
 void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst);
 
+// cholqr: the caller's next stage depends on the bases only through their projectors (KN), so
+// CholeskyQR2 may replace the Householder tree where the device certifies it (same span and
+// orthogonality to O(u), different rounding); otherwise the Householder tree, the reference's
+// factorization.
 template <typename T>
-void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst);
+void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst, bool cholqr = false);
 
 // returns iterations in *iter and the final error in *err (host values)
 template <typename T>
