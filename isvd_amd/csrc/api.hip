@@ -833,14 +833,14 @@ static void isvd_impl(const isvd_Param &param, const char *alg_s, const char *al
   }
   HIP_CHECK(hipEventRecord(ev[1], ctx().stream));
   stage_msg("done.\nOrthogonalizing ............... ");
-  // TS before KN: KN reads the bases only through their projectors, so CholeskyQR2 may factor
-  // them (see stage_orth_tsqr)
+  // before KN: KN reads the bases only through their projectors, so CholeskyQR2 may replace
+  // either orthogonalization (see stage_orth_tsqr / stage_orth_gramian)
   if constexpr (P::mixed) {
     if (ao[0] == 'T') stage_orth_tsqr_b(d, yst.get(), d.Nl, kn);
-    else stage_orth_gramian_b(d, yst.get(), d.Nl);
+    else stage_orth_gramian_b(d, yst.get(), d.Nl, kn);
   } else {
     if (ao[0] == 'T') stage_orth_tsqr<T>(d, yst.get(), d.Nl, kn);
-    else stage_orth_gramian<T>(d, yst.get(), d.Nl);
+    else stage_orth_gramian<T>(d, yst.get(), d.Nl, kn);
   }
   HIP_CHECK(hipEventRecord(ev[2], ctx().stream));
   stage_msg("done.\nIntegrating ................... ");

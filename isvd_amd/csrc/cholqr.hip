@@ -6,11 +6,13 @@
 // kappa(Y) <~ u^-1/2 (Y. Yamamoto, Y. Nakatsukasa, Y. Yanagisawa, T. Fukaya, "Roundoff error
 // analysis of the CholeskyQR2 algorithm", ETNA 44, 2015).  That condition is certified on the
 // device: a pivot that is not positive in either factorization, or ||Q1^T Q1 - I||_F > 1/2
-// after the first pass, makes the call return false with Y untouched, and the caller runs the
-// Householder TSQR tree instead.
+// after the first pass, makes the call return false with Y untouched, and the caller runs its
+// reference factorization instead.  The l x l steps run in FP64 also for FP32 data.
 #include "la.hpp"
+#include "mx.hpp"
 
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 namespace isvd {
@@ -115,34 +117,69 @@ void chol_inv(i64 n, i64 batch, const T *W, T *R, T *Rinv, int check, int *statu
 
 }  // namespace
 
-int tsqr_mode() {
-  const char *e = std::getenv("ISVD_TS");
+int cholqr_mode() {
+  const char *e = std::getenv("ISVD_CHOLQR");
   if (!e) return 0;
-  if (e[0] == 'h' || e[0] == 'H') return 1;
-  if (e[0] == 'c' || e[0] == 'C') return 2;
+  if (e[0] == '0') return 1;
+  if (e[0] == '1') return 2;
   return 0;
 }
+
+namespace {
+
+// l x l steps in FP64 for either data type: W (T) -> R, R^-1 (FP64), R^-1 also as T for the
+// product with the data
+template <typename T>
+void chol_step(i64 l, i64 batch, const T *w, double *w64, double *r, double *rinv64, T *rinv,
+               int check, int *status) {
+  const double *wd = w64;
+  if constexpr (std::is_same_v<T, double>) {
+    wd = w;
+  } else {
+    convert<double, float>(l, l * batch, w, l, w64, l);
+  }
+  chol_inv<double>(l, batch, wd, r, rinv64, check, status);
+  if constexpr (!std::is_same_v<T, double>) convert<float, double>(l, l * batch, rinv64, l, rinv, l);
+}
+
+}  // namespace
 
 template <typename T>
 bool cholqr2_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr, i64 sR) {
   ISVD_CHECK(l >= 1 && l <= 128 && m >= l, "CholeskyQR2: needs 1 <= l <= 128 and m >= l");
+  constexpr bool dbl = std::is_same_v<T, double>;
   const i64 ll = l * l, ldz = batch * l;
-  DBuf<T> w(batch * ll), r1(batch * ll), r2(batch * ll), rinv(batch * ll), z(m * ldz);
+  DBuf<T> w(batch * ll), z(m * ldz), rinv_t(dbl ? 0 : batch * ll);
+  DBuf<double> w64(dbl ? 0 : batch * ll), r1(batch * ll), r2(batch * ll), rinv(batch * ll);
   DBuf<int> st(2 * batch);
+  T *rinv_data;
+  if constexpr (dbl) rinv_data = rinv.get();
+  else rinv_data = rinv_t.get();
   // pass 1: W1 = Y^T Y, W1 = R1^T R1, Z^T = R1^-T Y^T
   gemm<T>('N', 'T', l, l, m, T(1), Y, ldy, Y, ldy, T(0), w.get(), l, batch, sY, sY, ll);
-  chol_inv<T>(l, batch, w.get(), r1.get(), rinv.get(), 0, st.get());
-  gemm<T>('T', 'N', l, m, l, T(1), rinv.get(), l, Y, ldy, T(0), z.get(), ldz, batch, ll, sY, l);
+  chol_step<T>(l, batch, w.get(), w64.get(), r1.get(), rinv.get(), rinv_data, 0, st.get());
+  gemm<T>('T', 'N', l, m, l, T(1), rinv_data, l, Y, ldy, T(0), z.get(), ldz, batch, ll, sY, l);
   // pass 2 on Z, certified by ||Z^T Z - I||_F <= 1/2
   gemm<T>('N', 'T', l, l, m, T(1), z.get(), ldz, z.get(), ldz, T(0), w.get(), l, batch, l, l, ll);
-  chol_inv<T>(l, batch, w.get(), r2.get(), rinv.get(), 1, st.get() + batch);
+  chol_step<T>(l, batch, w.get(), w64.get(), r2.get(), rinv.get(), rinv_data, 1,
+               st.get() + batch);
   std::vector<int> h(2 * batch);
   read_back(h.data(), st.get(), sizeof(int) * 2 * batch);
   for (int v : h)
     if (v) return false;
-  gemm<T>('T', 'N', l, m, l, T(1), rinv.get(), l, z.get(), ldz, T(0), Y, ldy, batch, ll, l, sY);
+  gemm<T>('T', 'N', l, m, l, T(1), rinv_data, l, z.get(), ldz, T(0), Y, ldy, batch, ll, l, sY);
   // R = R2 R1
-  if (R) gemm<T>('N', 'N', l, l, l, T(1), r2.get(), l, r1.get(), l, T(0), R, ldr, batch, ll, ll, sR);
+  if (R) {
+    if constexpr (dbl) {
+      gemm<double>('N', 'N', l, l, l, 1.0, r2.get(), l, r1.get(), l, 0.0, R, ldr, batch, ll, ll,
+                   sR);
+    } else {
+      gemm<double>('N', 'N', l, l, l, 1.0, r2.get(), l, r1.get(), l, 0.0, w64.get(), l, batch,
+                   ll, ll, ll);
+      for (i64 b = 0; b < batch; ++b)
+        convert<float, double>(l, l, w64.get() + b * ll, l, R + b * sR, ldr);
+    }
+  }
   return true;
 }
 

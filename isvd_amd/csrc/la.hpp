@@ -79,13 +79,13 @@ void tsqr_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr
 // CholeskyQR2 on the matrix cores: two passes of Gram, Cholesky and Y R^-1.  Returns false,
 // with Y and R untouched, unless the device certifies that the result is as accurate as
 // Householder QR (every pivot positive and ||Q1^T Q1 - I||_F <= 1/2 after the first pass); R's
-// diagonal is positive.
+// diagonal is positive.  The l x l Cholesky factorizations and inverses run in FP64.
 template <typename T>
 bool cholqr2_rowmajor(i64 m, i64 l, T *Y, i64 ldy, i64 sY, i64 batch, T *R, i64 ldr, i64 sR);
-// Which tall-skinny QR the stages take (ISVD_TS): 0 (unset) CholeskyQR2 where the caller allows
-// it and the device certifies it, else the Householder tree; 1 ("householder") the Householder
-// tree only; 2 ("cholqr") CholeskyQR2 wherever certified (test hook for the stage entry points).
-int tsqr_mode();
+// Where CholeskyQR2 replaces a reference factorization (ISVD_CHOLQR): 0 (unset) where the
+// caller allows it and the device certifies it; 1 ("0") nowhere; 2 ("1") also in the stage
+// entry points (test hook), wherever certified.
+int cholqr_mode();
 
 // l x l products (l <= 128): C := alpha op(A) op(B) + beta C + diag I; one short launch.
 template <typename T>

@@ -58,7 +58,11 @@ void stage_sketch_b(const Dims &d, char ordera, const uint16_t *a, i64 lda,
 }
 
 // @x@_orthogonalize_gramian.c:81-98, twice
-void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst) {
+void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst, bool cholqr) {
+  const int mode = cholqr_mode();
+  if (((cholqr && mode == 0) || mode == 2) &&
+      cholqr2_rowmajor<float>(d.m, d.l, yst, ldyst, d.l, d.N, nullptr, 0, 0))
+    return;
   const i64 l = d.l, N = d.N, m = d.m, Nl = d.Nl;
   DBuf<float> w32(l * l * N), u32(l * l * N), other(Nl * m);
   DBuf<double> w64(l * l * N), u64(l * l * N), s64(l * N);

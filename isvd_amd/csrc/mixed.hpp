@@ -16,7 +16,8 @@ void stage_sketch_b(const Dims &d, char ordera, const uint16_t *a, i64 lda,
 
 // Gramian orthogonalization applied twice in FP32 (Gram products and basis updates on FP32
 // MFMA; the l x l eigendecompositions in FP64)
-void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst);
+// (with cholqr: CholeskyQR2 where certified, as stage_orth_gramian)
+void stage_orth_gramian_b(const Dims &d, float *yst, i64 ldyst, bool cholqr = false);
 
 // FP32 tall-skinny QR (stage_orth_tsqr: CholeskyQR2 when cholqr and certified, else the
 // Householder tree)

@@ -260,9 +260,17 @@ void project_host_blocks(const Dims &d, char ordera, const T *a_host, i64 lda, c
   }
 }
 
-// Gramian orthogonalization (src/lib/libisvd/core/stage/@x@_orthogonalize_gramian.c:81-98).
+// Gramian orthogonalization (src/lib/libisvd/core/stage/@x@_orthogonalize_gramian.c:81-98):
+// Q_i = Y_i U_i S_i^-1 from W_i = Y_i^T Y_i = U_i S_i^2 U_i^T.  With cholqr (KN follows, see
+// stage_orth_tsqr), the Cholesky form of the same Gramian orthogonalization, done twice
+// (CholeskyQR2: Q_i = Y_i R_i^-1, the same span, another orthonormal basis of it), where the
+// device certifies it.
 template <typename T>
-void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst) {
+void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst, bool cholqr) {
+  const int mode = cholqr_mode();
+  if (((cholqr && mode == 0) || mode == 2) &&
+      cholqr2_rowmajor<T>(d.m, d.l, yst, ldyst, d.l, d.N, nullptr, 0, 0))
+    return;
   const i64 l = d.l, N = d.N, m = d.m;
   DBuf<T> w(l * d.Nl), u(l * d.Nl), s(l * N), ycopy(ldyst * m);
   // W_i := Y_i^T Y_i  (:81-84)
@@ -287,7 +295,7 @@ void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst) {
 // bases) by the Householder tree.
 template <typename T>
 void stage_orth_tsqr(const Dims &d, T *yst, i64 ldyst, bool cholqr) {
-  const int mode = tsqr_mode();
+  const int mode = cholqr_mode();
   if (((cholqr && mode == 0) || mode == 2) &&
       cholqr2_rowmajor<T>(d.m, d.l, yst, ldyst, d.l, d.N, nullptr, 0, 0))
     return;
@@ -733,7 +741,7 @@ void post_tsqr_z(const Dims &d, const T *qt, i64 ldqt, T *zt, T *s, T *ut, i64 l
   ISVD_CHECK(d.n >= l, "tall-skinny QR postprocessing needs n >= l (n = %lld, l = %lld)", d.n, l);
   DBuf<T> r(l * l), lmat(l * l), ul(l * l), vl(l * l);
   // Z (row-major n x l in zt) = Q_z R  <=>  Z^T = R^T Q_z^T = L Q_lq   (:199-208)
-  if (tsqr_mode() == 1 || !cholqr2_rowmajor<T>(d.n, l, zt, l, 0, 1, r.get(), l, 0))
+  if (cholqr_mode() == 1 || !cholqr2_rowmajor<T>(d.n, l, zt, l, 0, 1, r.get(), l, 0))
     tsqr_rowmajor<T>(d.n, l, zt, l, 0, 1, r.get(), l, 0);
   hipLaunchKernelGGL(transpose_small_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0,
                      ctx().stream, (int)l, (const T *)r.get(), l, lmat.get(), l);
@@ -781,7 +789,7 @@ void stage_post_symmetric(const Dims &d, char ordera, const T *a, i64 lda, const
                                             T *, i64, i64);                                    \
   template void project_host_blocks<T>(const Dims &, char, const T *, i64, const T *, i64, T *,  \
                                        i64);                                                   \
-  template void stage_orth_gramian<T>(const Dims &, T *, i64);                                \
+  template void stage_orth_gramian<T>(const Dims &, T *, i64, bool);                          \
   template void stage_orth_tsqr<T>(const Dims &, T *, i64, bool);                             \
   template void stage_integrate_kn<T>(const Dims &, const KnArgs &, const T *, i64, T *, i64,  \
                                       i64 *, double *);                                        \

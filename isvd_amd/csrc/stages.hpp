@@ -41,7 +41,7 @@ void project_host_blocks(const Dims &d, char ordera, const T *a_host, i64 lda, c
                          i64 ldqt, T *zt, i64 cols);
 
 template <typename T>
-void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst);
+void stage_orth_gramian(const Dims &d, T *yst, i64 ldyst, bool cholqr = false);
 
 // cholqr: the caller's next stage depends on the bases only through their projectors (KN), so
 // CholeskyQR2 may replace the Householder tree where the device certifies it (same span and
