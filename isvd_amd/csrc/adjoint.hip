@@ -235,7 +235,7 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
     LAUNCH_CHECK();
     read_back(h.data(), nrm.get(), sizeof(T) * 4 * k);
     RowMask rx{{0ull, 0ull}}, ry{{0ull, 0ull}};  // columns reverting to their first term
-    auto judge = [&](RowMask &act, RowMask &rev, std::vector<double> &prev, const T *term,
+    auto assess = [&](RowMask &act, RowMask &rev, std::vector<double> &prev, const T *term,
                      const T *sum) {
       for (int i = 0; i < (int)k; ++i) {
         if (!act.has(i)) continue;
@@ -247,8 +247,8 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
         if (grows && !done) rev.w[i >> 6] |= 1ull << (i & 63);
       }
     };
-    judge(ax, rx, prev_x, h.data(), h.data() + k);
-    judge(ay, ry, prev_y, h.data() + 2 * k, h.data() + 3 * k);
+    assess(ax, rx, prev_x, h.data(), h.data() + k);
+    assess(ay, ry, prev_y, h.data() + 2 * k, h.data() + 3 * k);
     if (any(rx)) rowscale<T>(k, m, s.get(), 0, x0.get(), k, xt.get(), k, T(0), rx);
     if (any(ry)) rowscale<T>(k, n, s.get(), 0, y0.get(), k, yt.get(), k, T(0), ry);
     if (trace_adjoint())

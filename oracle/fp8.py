@@ -3,7 +3,7 @@
 The reference (emfomy/isvd v1.3.2) has no reduced-precision path; this module states the
 storage format and the product that the library's e family implements, so that its kernels
 can be checked.  Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
-``cpu_baseline`` leg may import it, as the checker; the product path never calls it.
+``cpu_baseline`` leg may import it, to check results against; the product path never calls it.
 
 Format: OCP 8-bit floating point, E4M3 variant ("e4m3fn"): sign, 4 exponent bits (bias 7),
 3 mantissa bits; exponent field 0 holds subnormals m * 2^-9; 0x7F / 0xFF are NaN; no

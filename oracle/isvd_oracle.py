@@ -4,8 +4,8 @@ This module restates, stage by stage, the algorithm of the reference library
 emfomy/isvd v1.3.2 (a C99 library over BLAS/LAPACK/MPI) for ONE process (P = 1),
 so that the HIP implementation in ``isvd_amd`` can be checked against it.
 Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
-``bench.py`` may import it, and only as the checker; the product path never
-calls it.
+``bench.py`` may import it, and only to check results against; the product path
+never calls it.
 
 Arithmetic boundary.  The reference delegates every dense kernel to BLAS/LAPACK
 (``dgemm_``, ``dgesvd_``, ``dsyev_``, ``dgelqf_``/``dorglq_``, ``dgetrf_``/``dgetri_``,
@@ -101,11 +101,11 @@ def _sym_from_upper(x):
 def _qr_q(y):
     """Explicit Q of the Householder QR of y (m x l, m >= l) == (orglq of gelqf(y^T))^T."""
     pf = _pfx(y)
-    qr, tau, _work, info = getattr(_lp, pf + "geqrf")(y)
+    qr, tau, _lwork, info = getattr(_lp, pf + "geqrf")(y)
     if info != 0:
         raise RuntimeError(f"geqrf info={info}")
     r = np.triu(qr[: y.shape[1], :])
-    q, _work, info = getattr(_lp, pf + "orgqr")(qr, tau)
+    q, _lwork, info = getattr(_lp, pf + "orgqr")(qr, tau)
     if info != 0:
         raise RuntimeError(f"orgqr info={info}")
     return q[:, : y.shape[1]], r
