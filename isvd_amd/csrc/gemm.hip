@@ -112,6 +112,38 @@ __device__ inline void store_tile(const GemmArgs<T> &g,
                                   const typename Mfma<T>::acc_t (&acc)[FM][FN], i64 r0, i64 c0,
                                   int lane, int batch_id, int ks) {
   const int lr = lane & 15;
+  if constexpr (std::is_same_v<T, float>) {
+    // FP32 layout: a lane's four values are four consecutive rows of one column (rows
+    // 4 (lane / 16) + q of a 16-row fragment), stored as one 16-byte vector when aligned
+    const bool split = g.ksplit > 1;
+    T *Cb = split ? g.W + ((i64)batch_id * g.ksplit + ks) * g.m * g.n : g.C + (i64)batch_id * g.sC;
+    const i64 ldc = split ? g.m : g.ldc;
+    if (!g.ct && (ldc & 3) == 0 && (reinterpret_cast<uintptr_t>(Cb) & 15) == 0) {
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) {
+          const i64 r = r0 + a * 16 + Mfma<T>::row(lane, 0);
+          const i64 c = c0 + b * 16 + lr;
+          if (c >= g.n || r >= g.m) continue;
+          T *dst = Cb + r + c * ldc;
+          f4 v = acc[a][b];
+          if (!split) {
+            v = v * g.alpha;
+            if (g.beta != T(0)) {
+              for (int q = 0; q < 4; ++q)
+                if (r + q < g.m) v[q] += g.beta * dst[q];
+            }
+          }
+          if (r + 3 < g.m) {
+            *reinterpret_cast<f4 *>(dst) = v;
+          } else {
+            for (int q = 0; q < 4 && r + q < g.m; ++q) dst[q] = v[q];
+          }
+        }
+      return;
+    }
+  }
   if (g.ksplit == 1) {
     T *Cb = g.C + (i64)batch_id * g.sC;
 #pragma unroll
@@ -495,6 +527,157 @@ __global__ __launch_bounds__(dl::NT) void gemm_dl_kernel(GemmArgs<double> g) {
   store_tile<double, 4, 4>(g, acc, m0 + wm * 64, n0 + wn * 64, lane, batch_id, ks);
 }
 
+// The same kernel for FP32 operands that are both K-contiguous (op(A) = A^T, op(B) = B): a K
+// step of 32 floats fills the same 128-byte LDS rows (eight 16-byte chunks of four K values,
+// chunk c of row r at position c ^ swz(r)), so the stages, fills and raster are those of the
+// FP64 kernel byte for byte.  v_mfma_f32_16x16x4f32 takes one float per lane with lane l / 16
+// = s supplying K index k16 + 4 s + h to instruction h (h = 0..3) of a 16-byte chunk read, so a
+// half step (16 K values) is four instructions per fragment pair; the last instruction of each
+// step is deferred past the next barrier as in the FP64 kernel.
+namespace dl {
+constexpr int BK32 = 32;
+}  // namespace dl
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(dl::NT) void gemm_dl32_kernel(GemmArgs<float> g) {
+  using namespace dl;
+  using L = Layout<true, true>;  // 128-byte rows for both operands, as in the FP64 kernel
+  static_assert(L::A_BYTES == BM * BK32 * 4 && L::B_BYTES == BN * BK32 * 4, "row layout");
+  __shared__ __align__(16) unsigned char st0[L::STAGE], st1[L::STAGE], st2[L::STAGE];
+  auto stage = [&](auto k) -> unsigned char * {
+    constexpr int kk = decltype(k)::value;
+    if constexpr (kk == 0) return st0;
+    else if constexpr (kk == 1) return st1;
+    else return st2;
+  };
+  i64 tm, tn;
+  if (!tile_coords(g, BM, BN, tm, tn)) return;
+  const i64 m0 = tm * BM, n0 = tn * BN;
+  const int batch_id = blockIdx.y / g.ksplit;
+  const int ks = blockIdx.y % g.ksplit;
+  const i64 kbeg = (i64)ks * g.kchunk;
+  const i64 kend = (kbeg + g.kchunk < g.k) ? kbeg + g.kchunk : g.k;
+  const float *__restrict__ Ab = g.A + (i64)batch_id * g.sA;
+  const float *__restrict__ Bb = g.B + (i64)batch_id * g.sB;
+  const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
+  const int wm = wave & 1, wn = wave >> 1;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  const float *abase = Ab + m0 * g.lda + kbeg;
+  const float *bbase = Bb + n0 * g.ldb + kbeg;
+  unsigned off[FILLS], loff[FILLS];
+#pragma unroll
+  for (int u = 0; u < FILLS; ++u) {
+    const int f = 8 * u + wave;
+    if (u < A_FILLS / 8) {
+      const int row = 8 * f + (lane >> 3), c = (lane & 7) ^ swz(row);
+      const i64 gi = (m0 + row < g.m) ? row : g.m - 1 - m0;
+      off[u] = (unsigned)(gi * g.lda + 4 * c);
+      loff[u] = f * 1024;
+    } else {
+      const int fb = f - A_FILLS;
+      const int row = 8 * fb + (lane >> 3), c = (lane & 7) ^ swz(row);
+      const i64 gj = (n0 + row < g.n) ? row : g.n - 1 - n0;
+      off[u] = (unsigned)(gj * g.ldb + 4 * c);
+      loff[u] = L::A_BYTES + fb * 1024;
+    }
+  }
+  auto fill = [&](i64 kt, unsigned char *st) {
+    const unsigned base = lds_addr(st);
+    const float *ak = abase + kt * BK32, *bk = bbase + kt * BK32;
+#pragma unroll
+    for (int u = 0; u < FILLS; ++u)
+      lds_fill16((u < A_FILLS / 8 ? ak : bk) + off[u],
+                 __builtin_amdgcn_readfirstlane(base + loff[u]));
+  };
+  // LDS byte offset of op(A)[row][k] / op(B)[k][col] in a stage
+  auto a_off = [](int row, int k) { return row * 128 + (((k >> 2) ^ swz(row)) << 4) + (k & 3) * 4; };
+  auto b_off = [](int col, int k) {
+    return L::A_BYTES + col * 128 + (((k >> 2) ^ swz(col)) << 4) + (k & 3) * 4;
+  };
+  f4 acc[4][4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int b = 0; b < 4; ++b) acc[a][b] = f4{0.f, 0.f, 0.f, 0.f};
+
+  // fragments of one half step (16 K values): lane l / 16 = s holds K index k16 + 4 s + h for
+  // instruction h
+  typedef float Frag[4][4];
+  auto load = [&](const unsigned char *st, int k16, Frag &fa, Frag &fb) {
+    const int k = k16 + 4 * lk;  // this lane's K quad
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const f4v v = *reinterpret_cast<const f4v *>(st + a_off(wm * 64 + 16 * a + lr, k));
+      fa[a][0] = v.x; fa[a][1] = v.y; fa[a][2] = v.z; fa[a][3] = v.w;
+    }
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+      const f4v v = *reinterpret_cast<const f4v *>(st + b_off(wn * 64 + 16 * b + lr, k));
+      fb[b][0] = v.x; fb[b][1] = v.y; fb[b][2] = v.z; fb[b][3] = v.w;
+    }
+  };
+  // the 16 products of instruction h of a half step
+  auto mma = [&](const Frag &fa, const Frag &fb, int h) {
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+      for (int b = 0; b < 4; ++b)
+        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(fa[a][h], fb[b][h], acc[a][b], 0, 0, 0);
+  };
+
+  const i64 nkf = (kend - kbeg) / BK32;  // whole K steps only (dl32_eligible)
+  if (nkf > 0) fill(0, st0);
+  if (nkf > 1) fill(1, st1);
+  Frag fa0, fb0, fa1, fb1;
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int h = 0; h < 4; ++h) fa1[a][h] = fb1[a][h] = 0.f;  // nothing deferred before step 0
+  auto step = [&](i64 kt, auto k) {
+    constexpr int kk = decltype(k)::value;
+    if (kt + S - 2 < nkf) __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(FILLS * (S - 2)));
+    else __builtin_amdgcn_s_waitcnt(waitcnt_vm_lgkm0(0));
+    asm volatile("" ::"v"(fa1[0][3]), "v"(fa1[1][3]), "v"(fa1[2][3]), "v"(fa1[3][3]),
+                 "v"(fb1[0][3]), "v"(fb1[1][3]), "v"(fb1[2][3]), "v"(fb1[3][3]));
+    bare_barrier();
+    if (kt + S - 1 < nkf)
+      fill(kt + S - 1, stage(std::integral_constant<int, (kk + S - 1) % S>{}));
+    load(stage(k), 0, fa0, fb0);
+    __builtin_amdgcn_sched_barrier(0);
+    mma(fa1, fb1, 3);
+    mma(fa0, fb0, 0);
+    mma(fa0, fb0, 1);
+    mma(fa0, fb0, 2);
+    mma(fa0, fb0, 3);
+    load(stage(k), 16, fa1, fb1);
+    mma(fa1, fb1, 0);
+    mma(fa1, fb1, 1);
+    mma(fa1, fb1, 2);
+  };
+  for (i64 kb = 0; kb < nkf; kb += S) {
+    step(kb, std::integral_constant<int, 0>{});
+    if (kb + 1 < nkf) step(kb + 1, std::integral_constant<int, 1>{});
+    if (kb + 2 < nkf) step(kb + 2, std::integral_constant<int, 2>{});
+  }
+  mma(fa1, fb1, 3);  // the deferred products of the last step (zeros when none)
+  store_tile<float, 4, 4>(g, acc, m0 + wm * 64, n0 + wn * 64, lane, batch_id, ks);
+}
+
+// Whether gemm_dl32_kernel can take this product: both operands K-contiguous, every fill an
+// aligned 16-byte read inside the operands, and every K chunk whole K steps (the kernel has no
+// partial-step path; K = l or N l in the products that take it).
+inline bool dl32_eligible(const GemmArgs<float> &g, i64 batch) {
+  auto aligned = [](const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; };
+  if (g.k % dl::BK32 != 0 || (g.ksplit > 1 && g.kchunk % dl::BK32 != 0)) return false;
+  if (!aligned(g.A) || !aligned(g.B)) return false;
+  if ((g.lda & 3) || (g.ldb & 3)) return false;
+  if (batch > 1 && ((g.sA & 3) || (g.sB & 3))) return false;
+  if (g.lda >= (1 << 23) || g.ldb >= (1 << 23)) return false;
+  return true;
+}
+
 // Whether gemm_dl_kernel<KA, KB> can take this product (every fill is an aligned 16-byte
 // read inside the operands).
 inline bool dl_eligible(const GemmArgs<double> &g, bool KA, bool KB, i64 batch) {
@@ -624,6 +807,12 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
   if constexpr (std::is_same_v<T, double> && BM == dl::BM && BN == dl::BN && !(TA && TB)) {
     if (dl_eligible(g, TA, !TB, batch)) {
       hipLaunchKernelGGL((gemm_dl_kernel<TA, !TB>), grid, dim3(dl::NT), 0, ctx().stream, g);
+      done = true;
+    }
+  }
+  if constexpr (std::is_same_v<T, float> && BM == dl::BM && BN == dl::BN && TA && !TB) {
+    if (dl32_eligible(g, batch)) {
+      hipLaunchKernelGGL(gemm_dl32_kernel, grid, dim3(dl::NT), 0, ctx().stream, g);
       done = true;
     }
   }

@@ -7,7 +7,8 @@ Cholesky form of the Gramian orthogonalization (GR) in the FP64, FP32 and mixed 
 
 Shapes: 5 leaves (tree levels 5 -> 3 -> 2 -> 1, a lone last child at two levels), 12 leaves of
 250 rows (12 -> 6 -> 3 -> 2 -> 1), l = 37 (column groups of 4 that are partial, row accesses
-element by element), and one leaf taller than l.  Each block's Q must be orthonormal and span
+element by element), one leaf taller than l, and 1000 rows at l = 128 (an edge tile of the
+FP32 direct-to-LDS product Y R^-1 of CholeskyQR2).  Each block's Q must be orthonormal and span
 the block (equal projectors); the projector is fixed by the block alone, so a wrong reflector,
 a lost lone child, a misplaced column or a wrong triangular factor gives O(1) errors."""
 import numpy as np
@@ -22,7 +23,8 @@ TOL = {np.float64: 1e-11, np.float32: 2e-4}
 
 @pytest.mark.parametrize("path", ["householder", "cholqr"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32])
-@pytest.mark.parametrize("shape", [(1280, 128, 2), (3000, 100, 3), (777, 37, 3), (130, 128, 1)])
+@pytest.mark.parametrize("shape", [(1280, 128, 2), (3000, 100, 3), (777, 37, 3), (130, 128, 1),
+                                   (1000, 128, 3)])
 def test_tsqr_orthogonalization_matches_numpy_qr(torch_cuda, monkeypatch, path, dtype, shape):
     torch = torch_cuda
     monkeypatch.setenv("ISVD_CHOLQR", "0" if path == "householder" else "1")
@@ -72,7 +74,7 @@ def test_ill_conditioned_block_falls_back_to_householder(torch_cuda, monkeypatch
 
 
 @pytest.mark.parametrize("family", ["d", "s", "b"])
-@pytest.mark.parametrize("shape", [(1280, 128, 2), (777, 37, 3)])
+@pytest.mark.parametrize("shape", [(1280, 128, 2), (777, 37, 3), (1000, 128, 3)])
 def test_cholqr2_gramian_orthogonalization(torch_cuda, monkeypatch, family, shape):
     """GR in its Cholesky form (what the driver takes before KN): an orthonormal basis of each
     block's span, whatever basis of it (equal projectors with numpy's QR)."""
