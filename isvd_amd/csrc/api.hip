@@ -66,6 +66,7 @@ void ensure_init() {
   }
   HIP_CHECK(hipSetDevice(0));
   g_ctx.device = 0;
+  HIP_CHECK(hipDeviceGetAttribute(&g_ctx.cus, hipDeviceAttributeMultiprocessorCount, 0));
   // A blocking stream: work the caller queued on the legacy default stream (e.g. the kernels
   // that produced device-resident inputs) completes before the library's kernels start.
   HIP_CHECK(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamDefault));

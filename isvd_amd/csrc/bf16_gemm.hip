@@ -396,6 +396,8 @@ int run_bf16(char ta, char tb, i64 m, i64 n, i64 k, float alpha, const uint16_t 
     if (want > max_split) want = max_split;
     const size_t per = sizeof(float) * (size_t)(m * n);
     while (!W && want > 1 && per * (size_t)want > splitk_budget()) --want;
+    if (ctx().max_wgs > 0)
+      while (want > 1 && tiles * want > ctx().max_wgs) --want;
     ksplit = (int)(want < 1 ? 1 : want);
   }
   i64 kchunk = (k + ksplit - 1) / ksplit;

@@ -74,11 +74,23 @@ struct Ctx {
   hipStream_t aux = nullptr;
   hipEvent_t fork = nullptr, join = nullptr;
   int device = 0;
+  int cus = 256;  // compute units of the device
   Arena *arena = nullptr;  // non-null while a no-allocation entry point runs
+  // > 0 while a product runs beside a kernel that needs CUs of its own: the product splits K
+  // into no more chunks than keep its workgroups (one per CU) within this count
+  int max_wgs = 0;
 };
+
 
 Ctx &ctx();                // library context (created by isvd_Init_gpu or lazily)
 void ensure_init();         // selects device 0, creates the stream
+
+// Caps the workgroups of the enclosed products (Ctx::max_wgs) for the scope.
+struct GridCap {
+  int saved;
+  explicit GridCap(int cap) : saved(ctx().max_wgs) { ctx().max_wgs = cap; }
+  ~GridCap() { ctx().max_wgs = saved; }
+};
 
 // Device buffer owned by the caller's scope, allocated stream-ordered (or taken from the
 // active arena, which needs no release).

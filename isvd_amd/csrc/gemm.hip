@@ -785,6 +785,8 @@ void launch_cfg(GemmArgs<T> g, i64 batch) {
       ksplit = best;
     }
   }
+  if (ctx().max_wgs > 0)
+    while (ksplit > 1 && busy * batch * ksplit > ctx().max_wgs) --ksplit;
   // the partial sums must fit the space available for them (bounded under a scratch arena)
   {
     const size_t per = sizeof(T) * (size_t)(batch * g.m * g.n);

@@ -145,6 +145,11 @@ constexpr int kKnChainPart = 256;
 constexpr int kKnChainCoef = 66;
 constexpr int kKnChainBarWords = 8;
 inline i64 kn_chain_work_size(i64 l) { return 9 * l * l + kKnChainPart + kKnChainCoef; }
+// workgroups of one chain launch: two 16 x 16 tiles of the l x l result each
+inline int kn_chain_blocks(i64 l) {
+  const i64 t = (l + 15) / 16;
+  return (int)((t * t + 1) / 2);
+}
 // parts: nparts raw partial sums of B^T [B | Bg] (l x 2l each, dense, as gemm_partials writes
 // them); their sum / N is written to DDg (D | Dg, ld l) before the chain starts.
 template <typename T>

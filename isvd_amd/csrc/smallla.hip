@@ -749,7 +749,7 @@ void kn_chain(int l, T ninv, const T *parts, int nparts, T *DDg, const KnChainWo
   a.W3 = W3;
   a.scal = scal;
   a.bar = w.bar;
-  const int nwg = (tiles_m * tiles_m + 1) / 2;  // two tiles per workgroup
+  const int nwg = kn_chain_blocks(l);
   hipLaunchKernelGGL(kn_chain_kernel<T>, dim3(nwg), dim3(kChainThreads), 0, ctx().stream, a);
   LAUNCH_CHECK();
 }

@@ -409,23 +409,27 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
   auto gram = [&](i64 q) {
     nparts = gemm_partials<T>('T', 'N', l, 2 * l, Nl, B(q), Nl, B(q), Nl, parts.get(), want);
   };
-  // G_q := M Bg_q / N on the second stream, after the fork event (recorded once Bg_q is written)
+  // G_q := M Bg_q / N on the second stream, after the fork event (recorded once Bg_q is written).
+  // Its workgroups fill a CU's registers each: it leaves the chain's CUs free, or the chain,
+  // launched beside it, would wait for G's workgroups to finish before its own could start.
   auto g_product = [&](i64 q) {
     HIP_CHECK(hipStreamWaitEvent(ctx().aux, ctx().fork, 0));
     {
       OnStream on(ctx().aux);
+      GridCap cap(ctx().cus - kn_chain_blocks(l));
       m_apply(Bg(q), ninv, G(q));
     }
     HIP_CHECK(hipEventRecord(ctx().join, ctx().aux));
   };
-  // X_+ := X_c F + (B_c / N) C^-1 (image of :221-222); B_+ := B_c F + Bg_c C^-1 (:225-226);
-  // then Bg_+ := Bg_c F + G_c C^-1 after the join; the fork for the next G; the next Gram
+  // after the join, one batched launch: X_+ := X_c F + (B_c / N) C^-1 (image of :221-222),
+  // B_+ := B_c F + Bg_c C^-1 (:225-226), Bg_+ := Bg_c F + G_c C^-1 ([X | B], [B | Bg] and
+  // [Bg | G] are consecutive column blocks of the state); the fork for the next G; the next
+  // Gram.  (Updating X and B before the join would run them beside G on the few CUs G leaves
+  // to the chain.)
   auto updates = [&](i64 q) {
-    gemm_shortk<T>('N', 'N', Nl, l, 2 * l, T(1), X(q), Nl, w3.get(), 2 * l, T(0), X(q + 1), Nl,
-                   2, Nl * l, 2 * ll, Nl * l);
     HIP_CHECK(hipStreamWaitEvent(ctx().stream, ctx().join, 0));
-    gemm_shortk<T>('N', 'N', Nl, l, 2 * l, T(1), Bg(q), Nl, w3.get() + 4 * ll, 2 * l, T(0),
-                   Bg(q + 1), Nl);
+    gemm_shortk<T>('N', 'N', Nl, l, 2 * l, T(1), X(q), Nl, w3.get(), 2 * l, T(0), X(q + 1), Nl,
+                   3, Nl * l, 2 * ll, Nl * l);
     HIP_CHECK(hipEventRecord(ctx().fork, ctx().stream));
     gram(q + 1);
   };
