@@ -142,15 +142,15 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
                          const T *qt, i64 ldqt, const T *gs, const T *gut, i64 ldgut,
                          const T *gvt, i64 ldgvt, T *da, i64 ldda) {
   const i64 m = d.m, n = d.n, k = d.k, l = d.l, kl = l + k;
-  // products with A for k-row operands stored transposed (k x m and k x n, ld k):
-  //   xa: W (k x n) := X^T A,  xat: Z (k x m) := W A^T
-  auto xa = [&](const T *x, T *w) {
-    if (ordera == 'C') gemm<T>('N', 'N', k, n, m, T(1), x, k, a, lda, T(0), w, k);
-    else gemm<T>('N', 'T', k, n, m, T(1), x, k, a, lda, T(0), w, k);
+  // products with A for r-row operands stored transposed (r x m and r x n):
+  //   xa: W (r x n) := X^T A,  xat: Z (r x m) := W A^T
+  auto xa = [&](i64 r, const T *x, i64 ldx, T *w, i64 ldw) {
+    if (ordera == 'C') gemm<T>('N', 'N', r, n, m, T(1), x, ldx, a, lda, T(0), w, ldw);
+    else gemm<T>('N', 'T', r, n, m, T(1), x, ldx, a, lda, T(0), w, ldw);
   };
-  auto xat = [&](const T *w, T *z) {
-    if (ordera == 'C') gemm<T>('N', 'T', k, m, n, T(1), w, k, a, lda, T(0), z, k);
-    else gemm<T>('N', 'N', k, m, n, T(1), w, k, a, lda, T(0), z, k);
+  auto xat = [&](i64 r, const T *w, i64 ldw, T *z, i64 ldz) {
+    if (ordera == 'C') gemm<T>('N', 'T', r, m, n, T(1), w, ldw, a, lda, T(0), z, ldz);
+    else gemm<T>('N', 'N', r, m, n, T(1), w, ldw, a, lda, T(0), z, ldz);
   };
   // ---- U, S, V (all l triplets) recomputed from Qbar by the forward's postprocessing
   Dims dl = d;
@@ -162,16 +162,20 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
   else
     stage_post_gramian<T>(dl, ordera, a, lda, qt, ldqt, s.get(), ut_l.get(), l, vt_l.get(), l,
                           true, true);
-  // z (k x m) := z - (z U) U^T: the component outside span(U)
-  DBuf<T> cproj(k * l);
-  auto project = [&](T *z) {
-    gemm<T>('N', 'T', k, l, m, T(1), z, k, ut_l.get(), l, T(0), cproj.get(), k);
-    gemm<T>('N', 'N', k, m, l, T(-1), cproj.get(), k, ut_l.get(), l, T(1), z, k);
+  // z (r x m, r <= 2k) := z - (z U) U^T: the component outside span(U)
+  DBuf<T> cproj(2 * k * l);
+  auto project = [&](i64 r, T *z, i64 ldz) {
+    gemm<T>('N', 'T', r, l, m, T(1), z, ldz, ut_l.get(), l, T(0), cproj.get(), r);
+    gemm<T>('N', 'N', r, m, l, T(-1), cproj.get(), r, ut_l.get(), l, T(1), z, ldz);
   };
   // Pt = [U^T ; E^T]  ((l + k) x m),  Rt = [Mid V^T + [H^T ; 0] ; V_k^T]  ((l + k) x n)
   omatcopy<T>('N', l, m, T(1), ut_l.get(), l, pt.get(), kl);
   omatcopy<T>('N', k, n, T(1), vt_l.get(), l, rt.get() + l, kl);
-  DBuf<T> xt(k * m), xp(k * m), zt(k * m), yt(k * n), yp(k * n), wt(k * n);
+  // the series run half a step apart so that each step takes one product with A and one with
+  // A^T, each over 2k rows: zz = [X_p ; (P A Y_p)^T] (2k x m) and ww = [(A^T X_p)^T ; Y_p]
+  // (2k x n), ld 2k; xt, yt the sums; zt, wt scratch of k rows
+  const i64 k2 = 2 * k;
+  DBuf<T> xt(k * m), zt(k * m), yt(k * n), wt(k * n), zz(k2 * m), ww(k2 * n);
   {
     DBuf<T> j(l * l), kk(l * l), mid(l * l);
     // J[:, :k] = U^T gU, K[:, :k] = V^T gV (the remaining columns are zero)
@@ -201,11 +205,13 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
   // truncated-SVD backward, which treats the complement as carrying no singular value.
   DBuf<T> x0(k * m), y0(k * n);
   rowscale<T>(k, m, s.get(), -2, zt.get(), k, x0.get(), k, T(0));
-  rowscale<T>(k, m, s.get(), 0, x0.get(), k, xp.get(), k, T(0));
+  rowscale<T>(k, m, s.get(), 0, x0.get(), k, zz.get(), k2, T(0));
   rowscale<T>(k, m, s.get(), 0, x0.get(), k, xt.get(), k, T(0));
   rowscale<T>(k, n, s.get(), -2, wt.get(), k, y0.get(), k, T(0));
-  rowscale<T>(k, n, s.get(), 0, y0.get(), k, yp.get(), k, T(0));
   rowscale<T>(k, n, s.get(), 0, y0.get(), k, yt.get(), k, T(0));
+  // the first half step of the Y series: (P A Y_0)^T into the lower rows of zz
+  xat(k, y0.get(), k, zz.get() + k, k2);
+  project(k, zz.get() + k, k2);
   DBuf<T> nrm(4 * k);
   std::vector<T> h(4 * k);
   const double eps = (sizeof(T) == 8) ? 0x1p-52 : 0x1p-23;
@@ -214,22 +220,23 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
   std::vector<double> prev_x(k, INFINITY), prev_y(k, INFINITY);
   auto any = [](const RowMask &r) { return (r.w[0] | r.w[1]) != 0ull; };
   for (int it = 1; any(ax) || any(ay); ++it) {
-    xa(xp.get(), wt.get());
-    xat(wt.get(), zt.get());
-    project(zt.get());
-    rowscale<T>(k, m, s.get(), -2, zt.get(), k, xp.get(), k, T(0), ax, false);
-    rowscale<T>(k, m, s.get(), 0, xp.get(), k, xt.get(), k, T(1), ax);
-    xat(yp.get(), zt.get());
-    project(zt.get());
-    xa(zt.get(), wt.get());
-    rowscale<T>(k, n, s.get(), -2, wt.get(), k, yp.get(), k, T(0), ay, false);
-    rowscale<T>(k, n, s.get(), 0, yp.get(), k, yt.get(), k, T(1), ay);
+    // ww := [X_p ; (P A Y_p)^T] A: the X series' first half step and the Y series' second;
+    // Y_{p+1} = S^-2 (A^T P A Y_p) in place (rows of finished columns zeroed), Y += Y_{p+1}
+    xa(k2, zz.get(), k2, ww.get(), k2);
+    rowscale<T>(k, n, s.get(), -2, ww.get() + k, k2, ww.get() + k, k2, T(0), ay, false);
+    rowscale<T>(k, n, s.get(), 0, ww.get() + k, k2, yt.get(), k, T(1), ay);
+    // zz := P ww A^T: the X series' second half step and the Y series' first (for Y_{p+1});
+    // X_{p+1} = S^-2 (P A A^T X_p) in place, X += X_{p+1}
+    xat(k2, ww.get(), k2, zz.get(), k2);
+    project(k2, zz.get(), k2);
+    rowscale<T>(k, m, s.get(), -2, zz.get(), k2, zz.get(), k2, T(0), ax, false);
+    rowscale<T>(k, m, s.get(), 0, zz.get(), k2, xt.get(), k, T(1), ax);
     hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, m,
-                       (const T *)xp.get(), k, nrm.get());
+                       (const T *)zz.get(), k2, nrm.get());
     hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, m,
                        (const T *)xt.get(), k, nrm.get() + k);
     hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, n,
-                       (const T *)yp.get(), k, nrm.get() + 2 * k);
+                       (const T *)(ww.get() + k), k2, nrm.get() + 2 * k);
     hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, n,
                        (const T *)yt.get(), k, nrm.get() + 3 * k);
     LAUNCH_CHECK();
@@ -258,11 +265,11 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
                    (int)(2 * k));
   }
   // E^T = S X^T + (P A Y)^T into Pt rows l..l+k-1;  H^T = X^T A + S Y^T added to Rt rows 0..k-1
-  xat(yt.get(), zt.get());
-  project(zt.get());
+  xat(k, yt.get(), k, zt.get(), k);
+  project(k, zt.get(), k);
   rowscale<T>(k, m, s.get(), 0, zt.get(), k, pt.get() + l, kl, T(0));
   rowscale<T>(k, m, s.get(), 1, xt.get(), k, pt.get() + l, kl, T(1));
-  xa(xt.get(), wt.get());
+  xa(k, xt.get(), k, wt.get(), k);
   rowscale<T>(k, n, s.get(), 1, yt.get(), k, wt.get(), k, T(1));
   rowscale<T>(k, n, s.get(), 0, wt.get(), k, rt.get(), kl, T(1));
   // dA = Pt^T Rt  (m x n; for a row-major A the transpose n x m is formed instead)
