@@ -19,7 +19,9 @@ namespace isvd {
 
 namespace {
 
-constexpr int kCqThreads = 256;
+constexpr int kCqThreads = 1024;
+constexpr int kCqGroups = kCqThreads / 128;  // column groups of the trailing update
+constexpr int kCqLanes = kCqThreads / 128;   // lanes per row of the triangular inverse
 
 // One workgroup per l x l block: W (full symmetric storage, upper triangle read) -> R (upper
 // Cholesky factor, W = R^T R, lower part zero) and Rinv = R^-1 (upper).  With check, first
@@ -67,11 +69,16 @@ __global__ __launch_bounds__(kCqThreads) void chol_inv_kernel(int n, const T *__
     const T rkk = A[k + k * ld];
     for (int j = k + 1 + t; j < n; j += kCqThreads) A[k + j * ld] /= rkk;
     __syncthreads();
-    // A(i, j) -= R(k, i) R(k, j) for k < i <= j
-    const int w = n - k - 1;
-    for (int e = t; e < w * w; e += kCqThreads) {
-      const int i = k + 1 + e % w, j = k + 1 + e / w;
-      if (i <= j) A[i + j * ld] -= A[k + i * ld] * A[k + j * ld];
+    // A(i, j) -= R(k, i) R(k, j) for k < i <= j: a fixed row i per thread, columns strided by
+    // the number of column groups
+    {
+      const int i = k + 1 + (t & 127);
+      if (i < n) {
+        const T rki = A[k + i * ld];
+        for (int j = i + ((t >> 7) + kCqGroups - ((i - k - 1) % kCqGroups)) % kCqGroups; j < n;
+             j += kCqGroups)
+          A[i + j * ld] -= rki * A[k + j * ld];
+      }
     }
     __syncthreads();
   }
@@ -85,15 +92,20 @@ __global__ __launch_bounds__(kCqThreads) void chol_inv_kernel(int n, const T *__
   // holds its inverse T): x = R(0:j, j);  Rinv(0:j, j) = -T x / R(j, j);  Rinv(j, j) = 1 / R(j, j)
   // (the pivot is copied in the first phase: thread 0 replaces A(j, j) in the second, where no
   // other thread reads it)
+  // Each row's sum is split over kCqLanes consecutive lanes (k = i + q, i + q + kCqLanes, ...),
+  // added in a fixed tree.
   for (int j = 0; j < n; ++j) {
     for (int i = t; i < j; i += kCqThreads) tmp[i] = A[i + j * ld];
     if (t == 0) piv = A[j + j * ld];
     __syncthreads();
     const T rjj = piv;
-    for (int i = t; i < j; i += kCqThreads) {
+    {
+      const int i = t / kCqLanes, q = t % kCqLanes;
       T s = 0;
-      for (int k = i; k < j; ++k) s += A[i + k * ld] * tmp[k];
-      A[i + j * ld] = -s / rjj;
+      if (i < j)
+        for (int k = i + q; k < j; k += kCqLanes) s += A[i + k * ld] * tmp[k];
+      for (int off = kCqLanes / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+      if (i < j && q == 0) A[i + j * ld] = -s / rjj;
     }
     if (t == 0) A[j + j * ld] = T(1) / rjj;
     __syncthreads();

@@ -27,20 +27,39 @@ __global__ void fill_kernel(i64 m, i64 n, T value, T *A, i64 lda, T diag, i64 sA
   }
 }
 
-// one workgroup, fixed reduction tree -> bitwise reproducible
+// Two passes with a fixed reduction tree (bitwise reproducible for a given n): workgroup b sums
+// the chunk [b chunk, (b + 1) chunk) into part[b]; one workgroup then sums the parts in order.
+constexpr int kDotThreads = 256, kDotParts = 512;
+
 template <typename T>
-__global__ __launch_bounds__(1024) void dot_kernel(i64 n, const T *x, const T *y, T *out) {
-  __shared__ T red[16];
-  T s = 0;
-  for (i64 e = threadIdx.x; e < n; e += 1024) s += x[e] * y[e];
+__device__ T block_sum(T s, T *red) {
   for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    T t = 0;
-    for (int w = 0; w < 16; ++w) t += red[w];
-    out[0] = t;
-  }
+  T t = 0;
+  for (int w = 0; w < kDotThreads / 64; ++w) t += red[w];
+  return t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kDotThreads) void dot_parts_kernel(i64 n, i64 chunk, const T *x,
+                                                                const T *y, T *part) {
+  __shared__ T red[kDotThreads / 64];
+  const i64 beg = (i64)blockIdx.x * chunk, end = (beg + chunk < n) ? beg + chunk : n;
+  T s = 0;
+  for (i64 e = beg + threadIdx.x; e < end; e += kDotThreads) s += x[e] * y[e];
+  const T t = block_sum(s, red);
+  if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kDotThreads) void dot_final_kernel(int nparts, const T *part,
+                                                                T *out) {
+  __shared__ T red[kDotThreads / 64];
+  T s = 0;
+  for (int e = threadIdx.x; e < nparts; e += kDotThreads) s += part[e];
+  const T t = block_sum(s, red);
+  if (threadIdx.x == 0) out[0] = t;
 }
 
 }  // namespace
@@ -69,7 +88,21 @@ void fill(i64 m, i64 n, T value, T *A, i64 lda, T diag, i64 batch, i64 sA) {
 
 template <typename T>
 void dot(i64 n, const T *x, const T *y, T *result) {
-  hipLaunchKernelGGL(dot_kernel<T>, dim3(1), dim3(1024), 0, ctx().stream, n, x, y, result);
+  i64 chunk = (n + kDotParts - 1) / kDotParts;
+  if (chunk < 4 * kDotThreads) chunk = 4 * kDotThreads;
+  const int nparts = (int)((n + chunk - 1) / chunk);
+  if (nparts <= 1) {
+    hipLaunchKernelGGL(dot_parts_kernel<T>, dim3(1), dim3(kDotThreads), 0, ctx().stream, n,
+                       n > 0 ? n : 1, x, y, result);
+    LAUNCH_CHECK();
+    return;
+  }
+  DBuf<T> part(nparts);
+  hipLaunchKernelGGL(dot_parts_kernel<T>, dim3(nparts), dim3(kDotThreads), 0, ctx().stream, n,
+                     chunk, x, y, part.get());
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(dot_final_kernel<T>, dim3(1), dim3(kDotThreads), 0, ctx().stream, nparts,
+                     (const T *)part.get(), result);
   LAUNCH_CHECK();
 }
 

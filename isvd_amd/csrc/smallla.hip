@@ -145,14 +145,21 @@ __global__ __launch_bounds__(kGjThreads) void gj_inverse_lds_kernel(int n, T *A,
     for (int r = tid; r < n; r += kGjThreads) colj[r] = S[r * ld + j];
     __syncthreads();
     const T ipv = T(1) / rowj[j];
-    for (int e = tid; e < n * n; e += kGjThreads) {
-      const int r = e / n, c = e % n;
-      T v;
-      if (r == j)
-        v = (c == j) ? ipv : rowj[c] * ipv;
-      else
-        v = (c == j) ? -colj[r] * ipv : S[r * ld + c] - colj[r] * ipv * rowj[c];
-      S[r * ld + c] = v;
+    // a fixed column per thread, rows strided by the number of row groups (n <= 128)
+    {
+      constexpr int kGroups = kGjThreads / 128;
+      const int c = tid & 127;
+      if (c < n) {
+        const T rc = rowj[c];
+        for (int r = tid >> 7; r < n; r += kGroups) {
+          T v;
+          if (r == j)
+            v = (c == j) ? ipv : rc * ipv;
+          else
+            v = (c == j) ? -colj[r] * ipv : S[r * ld + c] - colj[r] * ipv * rc;
+          S[r * ld + c] = v;
+        }
+      }
     }
     __syncthreads();
   }

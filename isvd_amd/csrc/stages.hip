@@ -534,7 +534,7 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
   const T ninv = T(1) / T(d.N);
   const T half_ninv = T(0.5) / T(d.N);
   DBuf<T> xq(Nl * l), xqp(Nl * l), xx(Nl * l), xxp(Nl * l), bc(Nl * l), bp(Nl * l),
-      bgc(Nl * l), gq(Nl * l), d1(Nl * l), d2(Nl * l), md1(Nl * l), md2(Nl * l);
+      bgc(Nl * l), gq(Nl * l), d1(Nl * l), d2(Nl * l), md1(Nl * l);
   DBuf<T> dc(l * l), dgc(l * l), zm(l * l), si(l * l), fcb(l * l), fgcb(l * l), scal(4);
   DBuf<int> info(1);
   // init (:201-235)
@@ -621,16 +621,13 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
     hipLaunchKernelGGL(sub_kernel<T>, dim3(grid_for(Nl * l)), dim3(256), 0, ctx().stream,
                        Nl * l, (const T *)xx.get(), (const T *)xxp.get(), d2.get());
     LAUNCH_CHECK();
-    // t (:334-342) in sketch coordinates: <D_a, D_b> = <d_a, M d_b>
-    gemm_ct<T>('T', 'N', l, Nl, Nl, T(1), d1.get(), Nl, M, Nl, T(0), md1.get(), Nl);
-    gemm_ct<T>('T', 'N', l, Nl, Nl, T(1), d2.get(), Nl, M, Nl, T(0), md2.get(), Nl);
-    if (it % 2) {
-      dot<T>(Nl * l, d1.get(), md1.get(), scal.get());
-      dot<T>(Nl * l, d1.get(), md2.get(), scal.get() + 1);
-    } else {
-      dot<T>(Nl * l, d2.get(), md1.get(), scal.get());
-      dot<T>(Nl * l, d2.get(), md2.get(), scal.get() + 1);
-    }
+    // t (:334-342) in sketch coordinates: <D_a, D_b> = <d_a, M d_b>, M symmetric.  Odd
+    // iterations take <D1, D1>, <D1, D2>, even ones <D2, D1>, <D2, D2>: both are
+    // [<d1, v>, <d2, v>] with v = M d1 (odd) or M d2 (even), one product with M
+    gemm_ct<T>('T', 'N', l, Nl, Nl, T(1), (it % 2) ? d1.get() : d2.get(), Nl, M, Nl, T(0),
+               md1.get(), Nl);
+    dot<T>(Nl * l, d1.get(), md1.get(), scal.get());
+    dot<T>(Nl * l, d2.get(), md1.get(), scal.get() + 1);
     T t[2];
     read_back(t, scal.get(), 2 * sizeof(T));
     taug = std::fabs((double)t[0] / (double)t[1]);
