@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kGjThreads) void gj_inverse_lds_kernel(int n, T *A,
   const int ld = n + 1;
   T *S = reinterpret_cast<T *>(smem_raw);  // n x ld, row-major
   __shared__ T rowj[128], colj[128];
-  __shared__ int ipiv[128], pos[128];
+  __shared__ int ipiv[128], pos[128], colidx[128];
   __shared__ int piv_sh, bad_sh;
   const int tid = threadIdx.x;
   for (int e = tid; e < n * n; e += kGjThreads) {
@@ -168,7 +168,7 @@ __global__ __launch_bounds__(kGjThreads) void gj_inverse_lds_kernel(int n, T *A,
     return;
   }
   if (tid == 0) {
-    int idx[128];
+    int *idx = colidx;  // in LDS: a private array would live in scratch memory
     for (int q = 0; q < n; ++q) idx[q] = q;
     for (int q = n - 1; q >= 0; --q) {
       const int t = idx[q];
