@@ -14,6 +14,10 @@ __device__ __forceinline__ double dpp_mov(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
 }
 template <int CTRL>
+__device__ __forceinline__ int dpp_mov(int v) {
+  return __builtin_amdgcn_update_dpp(0, v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
 __device__ __forceinline__ float dpp_mov(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }

@@ -6,6 +6,7 @@
 //  - kn_chain: the l x l part of a Kolmogorov-Nagumo iteration in one launch (C and C^-1 of
 //    (I/2 + (I/4 - Z)^{1/2})^{1/2} by a Taylor series or coupled Newton-Schulz square roots);
 //  - inv_small: Gauss-Jordan inverse with partial pivoting, matrix held in LDS.
+#include "dpp.hpp"
 #include "la.hpp"
 
 #include <cmath>
@@ -119,10 +120,29 @@ __global__ __launch_bounds__(kGjThreads) void gj_inverse_lds_kernel(int n, T *A,
         const T v = fabs(S[r * ld + j]);
         if (v > best) { best = v; bi = r; }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const T ob = __shfl_xor(best, off, 64);
-        const int oi = __shfl_xor(bi, off, 64);
+      // (largest, then smallest index) is associative and commutative: within each 16-lane
+      // row over DPP (pairs, quads, half rows, rows), then the four rows' results by readlane
+      auto take = [&](T ob, int oi) {
         if (ob > best || (ob == best && oi < bi)) { best = ob; bi = oi; }
+      };
+      take(dpp_mov<0xB1>(best), dpp_mov<0xB1>(bi));
+      take(dpp_mov<0x4E>(best), dpp_mov<0x4E>(bi));
+      take(dpp_mov<0x141>(best), dpp_mov<0x141>(bi));
+      take(dpp_mov<0x140>(best), dpp_mov<0x140>(bi));
+      {
+        T rb[3];
+        int ri[3];
+        const long long bits = __double_as_longlong((double)best);
+        const int lo = (int)(bits & 0xffffffffll), hi = (int)(bits >> 32);
+#pragma unroll
+        for (int q = 1; q < 4; ++q) {
+          const unsigned l32 = (unsigned)__builtin_amdgcn_readlane(lo, 16 * q);
+          const long long h32 = __builtin_amdgcn_readlane(hi, 16 * q);
+          rb[q - 1] = (T)__longlong_as_double((h32 << 32) | l32);
+          ri[q - 1] = __builtin_amdgcn_readlane(bi, 16 * q);
+        }
+        if (tid == 0)
+          for (int q = 0; q < 3; ++q) take(rb[q], ri[q]);
       }
       if (tid == 0) {
         piv_sh = bi;
