@@ -118,21 +118,56 @@ void rowscale(i64 k, i64 cols, const T *s, int e, const T *G, i64 ldg, T *P, i64
   rowscale<T>(k, cols, s, e, G, ldg, P, ldp, beta, all_rows(k));
 }
 
-// out[i] := sum over columns of P[i, c]^2 for the k rows of P (ld ldp); one workgroup per row,
-// fixed reduction order
+// out[i] := sum over columns of P[i, c]^2 for the k <= 128 rows of P (ld ldp), in a fixed order
+// (bitwise reproducible): workgroup b sweeps the columns of chunk b with k consecutive lanes per
+// column (coalesced) and writes one partial sum per row; a second pass adds the chunks in order.
+constexpr int kNormThreads = 256, kNormChunks = 128;
+
 template <typename T>
-__global__ __launch_bounds__(256) void rownorm2_kernel(i64 cols, const T *P, i64 ldp, T *out) {
-  __shared__ T red[4];
-  const int i = blockIdx.x;
-  T part = 0;
-  for (i64 c = threadIdx.x; c < cols; c += 256) {
-    const T v = P[i + c * ldp];
-    part += v * v;
+__global__ __launch_bounds__(kNormThreads) void rownorm2_parts_kernel(int k, i64 cols, i64 chunk,
+                                                                      const T *P, i64 ldp,
+                                                                      T *part) {
+  __shared__ T red[kNormThreads];
+  const int t = threadIdx.x, lanes = kNormThreads / k;
+  const i64 beg = (i64)blockIdx.x * chunk, end = (beg + chunk < cols) ? beg + chunk : cols;
+  T s = 0;
+  if (t < k * lanes) {
+    const int i = t % k, lane = t / k;
+    for (i64 c = beg + lane; c < end; c += lanes) {
+      const T v = P[i + c * ldp];
+      s += v * v;
+    }
   }
-  for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = part;
+  red[t] = s;
   __syncthreads();
-  if (threadIdx.x == 0) out[i] = red[0] + red[1] + red[2] + red[3];
+  if (t < k) {
+    T r = 0;
+    for (int q = 0; q < lanes; ++q) r += red[t + q * k];
+    part[(i64)blockIdx.x * k + t] = r;
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(128) void rownorm2_final_kernel(int k, int nchunks, const T *part,
+                                                             T *out) {
+  const int i = threadIdx.x;
+  if (i >= k) return;
+  T r = 0;
+  for (int b = 0; b < nchunks; ++b) r += part[(i64)b * k + i];
+  out[i] = r;
+}
+
+template <typename T>
+void rownorm2(int k, i64 cols, const T *P, i64 ldp, T *out, T *part) {
+  i64 chunk = (cols + kNormChunks - 1) / kNormChunks;
+  if (chunk < 1) chunk = 1;
+  const int nchunks = (int)((cols + chunk - 1) / chunk);
+  hipLaunchKernelGGL(rownorm2_parts_kernel<T>, dim3((unsigned)(nchunks > 0 ? nchunks : 1)),
+                     dim3(kNormThreads), 0, ctx().stream, k, cols, chunk, P, ldp, part);
+  LAUNCH_CHECK();
+  hipLaunchKernelGGL(rownorm2_final_kernel<T>, dim3(1), dim3(128), 0, ctx().stream, k,
+                     nchunks > 0 ? nchunks : 1, (const T *)part, out);
+  LAUNCH_CHECK();
 }
 
 }  // namespace
@@ -212,7 +247,7 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
   // the first half step of the Y series: (P A Y_0)^T into the lower rows of zz
   xat(k, y0.get(), k, zz.get() + k, k2);
   project(k, zz.get() + k, k2);
-  DBuf<T> nrm(4 * k);
+  DBuf<T> nrm(4 * k), nrm_part(kNormChunks * k);
   std::vector<T> h(4 * k);
   const double eps = (sizeof(T) == 8) ? 0x1p-52 : 0x1p-23;
   constexpr int kMaxTerms = 300;
@@ -231,15 +266,10 @@ void stage_isvd_backward(const Dims &d, bool tsqr, char ordera, const T *a, i64 
     project(k2, zz.get(), k2);
     rowscale<T>(k, m, s.get(), -2, zz.get(), k2, zz.get(), k2, T(0), ax, false);
     rowscale<T>(k, m, s.get(), 0, zz.get(), k2, xt.get(), k, T(1), ax);
-    hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, m,
-                       (const T *)zz.get(), k2, nrm.get());
-    hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, m,
-                       (const T *)xt.get(), k, nrm.get() + k);
-    hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, n,
-                       (const T *)(ww.get() + k), k2, nrm.get() + 2 * k);
-    hipLaunchKernelGGL(rownorm2_kernel<T>, dim3((unsigned)k), dim3(256), 0, ctx().stream, n,
-                       (const T *)yt.get(), k, nrm.get() + 3 * k);
-    LAUNCH_CHECK();
+    rownorm2<T>((int)k, m, (const T *)zz.get(), k2, nrm.get(), nrm_part.get());
+    rownorm2<T>((int)k, m, (const T *)xt.get(), k, nrm.get() + k, nrm_part.get());
+    rownorm2<T>((int)k, n, (const T *)(ww.get() + k), k2, nrm.get() + 2 * k, nrm_part.get());
+    rownorm2<T>((int)k, n, (const T *)yt.get(), k, nrm.get() + 3 * k, nrm_part.get());
     read_back(h.data(), nrm.get(), sizeof(T) * 4 * k);
     RowMask rx{{0ull, 0ull}}, ry{{0ull, 0ull}};  // columns reverting to their first term
     auto assess = [&](RowMask &act, RowMask &rev, std::vector<double> &prev, const T *term,
