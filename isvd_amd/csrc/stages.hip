@@ -3,6 +3,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 #include "la.hpp"
@@ -535,8 +536,10 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
   const T half_ninv = T(0.5) / T(d.N);
   DBuf<T> xq(Nl * l), xqp(Nl * l), xx(Nl * l), xxp(Nl * l), bc(Nl * l), bp(Nl * l),
       bgc(Nl * l), gq(Nl * l), d1(Nl * l), d2(Nl * l), md1(Nl * l);
-  DBuf<T> dc(l * l), dgc(l * l), zm(l * l), si(l * l), fcb(l * l), fgcb(l * l), scal(4);
-  DBuf<int> info(1);
+  // scal: [0] mu, [1] phi, [2] phit, [3] the inverse's pivot status (an int in the slot),
+  // [4..5] t; the values the host needs at one point are read back together
+  DBuf<T> dc(l * l), dgc(l * l), zm(l * l), si(l * l), fcb(l * l), fgcb(l * l), scal(6);
+  int *info = reinterpret_cast<int *>(scal.get() + 3);
   // init (:201-235)
   fill<T>(Nl, l, T(0), xq.get(), Nl, T(1));
   omatcopy<T>('N', Nl, l, T(1), M, Nl, bc.get(), Nl);
@@ -569,7 +572,7 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
       hipLaunchKernelGGL(sym_scale_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
                          (int)l, (const T *)zm.get(), si.get(), T(-0.5), T(-2) * it_ * it_);
       LAUNCH_CHECK();
-      inv_small<T>(l, si.get(), l, info.get());
+      inv_small<T>(l, si.get(), l, info);
       hipLaunchKernelGGL(sym_scale_kernel<T>, dim3(grid_for(l * l)), dim3(256), 0, ctx().stream,
                          (int)l, (const T *)si.get(), fgc, T(-2) * it_, T(0));
       LAUNCH_CHECK();
@@ -581,9 +584,11 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
       gemm<T>('N', 'N', Nl, l, l, T(1), bc.get(), Nl, fc, l, T(0), bp.get(), Nl);
       gemm<T>('N', 'N', Nl, l, l, T(1), bgc.get(), Nl, fgc, l, T(1), bp.get(), Nl);
       dot<T>(Nl * l, bp.get(), bp.get(), scal.get() + 2);
+      T hv[2];
+      read_back(hv, scal.get() + 2, sizeof(hv));
       int h_info = 0;
-      read_back(&h_info, info.get(), sizeof(int));
-      phit = (double)half_ninv * (double)read_scalar(scal.get() + 2);
+      std::memcpy(&h_info, &hv[1], sizeof(int));
+      phit = (double)half_ninv * (double)hv[0];
       ISVD_CHECK(h_info == 0, "Wen-Yin step matrix is singular (pivot %d)", h_info);
       if (trace_on())
         std::fprintf(stderr, "isvd WY iter %lld trip %lld tau %.17g phit %.17g phi %.17g\n", it,
@@ -605,15 +610,8 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
     hipLaunchKernelGGL(wy_mu_kernel<T>, dim3(1), dim3(256), 0, ctx().stream, (int)l,
                        (const T *)dgc.get(), (const T *)dc.get(), scal.get());
     LAUNCH_CHECK();
-    mu = (double)read_scalar(scal.get());
-    if (trace_on()) std::fprintf(stderr, "isvd WY iter %lld mu %.17g\n", it, mu);
-    // convergence (:312-319)
-    if (mu < args.tol * args.tol) break;
-    if (it == args.maxit) {
-      ++it;
-      break;
-    }
-    // X+ := G+ - Q+ D+ (:324-325); Delta1, Delta2 (:328-329)
+    // X+ := G+ - Q+ D+ (:324-325); Delta1, Delta2 (:328-329), queued before mu is read back so
+    // that mu and t come back together (after the last iteration they go unused)
     omatcopy<T>('N', Nl, l, T(1), gq.get(), Nl, xxp.get(), Nl);
     gemm<T>('N', 'N', Nl, l, l, T(-1), xqp.get(), Nl, dc.get(), l, T(1), xxp.get(), Nl);
     hipLaunchKernelGGL(sub_kernel<T>, dim3(grid_for(Nl * l)), dim3(256), 0, ctx().stream,
@@ -626,10 +624,19 @@ void integrate_wy_coords(const Dims &d, const WyArgs &args, const T *M, T *x_out
     // [<d1, v>, <d2, v>] with v = M d1 (odd) or M d2 (even), one product with M
     gemm_ct<T>('T', 'N', l, Nl, Nl, T(1), (it % 2) ? d1.get() : d2.get(), Nl, M, Nl, T(0),
                md1.get(), Nl);
-    dot<T>(Nl * l, d1.get(), md1.get(), scal.get());
-    dot<T>(Nl * l, d2.get(), md1.get(), scal.get() + 1);
-    T t[2];
-    read_back(t, scal.get(), 2 * sizeof(T));
+    dot<T>(Nl * l, d1.get(), md1.get(), scal.get() + 4);
+    dot<T>(Nl * l, d2.get(), md1.get(), scal.get() + 5);
+    T hs[6];
+    read_back(hs, scal.get(), sizeof(hs));
+    mu = (double)hs[0];
+    if (trace_on()) std::fprintf(stderr, "isvd WY iter %lld mu %.17g\n", it, mu);
+    // convergence (:312-319)
+    if (mu < args.tol * args.tol) break;
+    if (it == args.maxit) {
+      ++it;
+      break;
+    }
+    const T t[2] = {hs[4], hs[5]};
     taug = std::fabs((double)t[0] / (double)t[1]);
     if (taug < args.taumin) taug = args.taumin;
     if (taug > args.taumax) taug = args.taumax;
