@@ -273,7 +273,12 @@ enum ChainEpi {
   kEpiErrSq = 32,   // part[tile] := sum of (v - delta)^2
   kEpiW3Low = 64,   // W3 lower halves := [ninv v | v | v]
   kEpiW3Up = 128,   // W3 upper halves := [v | v | v]
+  kEpiTopLast = 256,  // (with kEpiTop, one block) the top blocks are C and C^-1: part[tile +
+                      // kPartErr] := sum of (C - delta)^2, W3 lower halves := [ninv C^-1 | ..]
 };
+// part[] holds ||Z2||_F partials read after the Z2 round; the error partials of a round that
+// runs while some workgroups may still read those go to part[kPartErr ..]
+constexpr int kPartErr = 64;
 
 template <typename T>
 struct ChainProd {
@@ -425,8 +430,17 @@ __device__ void chain_round(const ChainArgs<T> &a, const ChainProd<T> p0, const 
         }
         if (epi & kEpiStart2) cst(p.out2 + e, T(0.25) * (T(5) * dl - cld(p.Cin + e)));
         if (epi & kEpiTop) {
-          cst(p.out2 + e, block_at(a, p.coef, e, dl, v));
-          cst(p.out3 + e, block_at(a, p.coef + kSeriesTerms, e, dl, v));
+          const T cv = block_at(a, p.coef, e, dl, v);
+          const T gv = block_at(a, p.coef + kSeriesTerms, e, dl, v);
+          cst(p.out2 + e, cv);
+          cst(p.out3 + e, gv);
+          if (epi & kEpiTopLast) {  // read only after the kernel: plain stores
+            sq += (cv - dl) * (cv - dl);
+            T *w3 = a.W3 + l + r + (i64)c * 2 * l;
+            w3[0] = a.ninv * gv;
+            w3[2 * ll] = gv;
+            w3[4 * ll] = gv;
+          }
         }
         if (epi & kEpiNormSq) sq += v * v;
         if (epi & kEpiErrSq) sq += (v - dl) * (v - dl);
@@ -438,9 +452,9 @@ __device__ void chain_round(const ChainArgs<T> &a, const ChainProd<T> p0, const 
         }
       }
     }
-    if (epi & (kEpiNormSq | kEpiErrSq)) {
+    if (epi & (kEpiNormSq | kEpiErrSq | kEpiTopLast)) {
       for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
-      if (lane == 0) cst(a.part + tile, sq);
+      if (lane == 0) cst(a.part + tile + ((epi & kEpiTopLast) ? kPartErr : 0), sq);
     }
   }
   __syncthreads();  // red is reused by the next round
@@ -530,37 +544,17 @@ __device__ void chain_body(const ChainArgs<T> &a, typename SM<T>::acc (*red)[64]
         p3.coef = a.coef + 4 * (t - 1);
         p3.out2 = a.Y[cur];
         p3.out3 = a.W[cur];
-        const ChainProd<T> p4 = prod(a.Z2, a.Z2, a.Z4, false, T(1), T(0));
-        round2<T>(a, p3, p4, red);
+        if (t == 1) {
+          // C and C^-1 are the top blocks: ||C - I||_F partials and the C^-1 halves of W3 come
+          // with them, and Z^4 is not needed
+          p3.epi |= kEpiTopLast;
+          round1<T>(a, p3, red);
+        } else {
+          const ChainProd<T> p4 = prod(a.Z2, a.Z2, a.Z4, false, T(1), T(0));
+          round2<T>(a, p3, p4, red);
+        }
       }
       if (!chain_barrier(a.bar)) goto timeout;
-      if (t == 1) {
-        // C and C^-1 are the top blocks: ||C - I||_F and the C^-1 halves of W3, elementwise
-        const int tile = 2 * blockIdx.x + (threadIdx.x >> 8);
-        const int tm = tile % a.tiles_m, tn = tile / a.tiles_m;
-        T sq = 0;
-        if (tile < ntile) {
-          const int i = tm * 16 + (threadIdx.x & 15), j = tn * 16 + ((threadIdx.x >> 4) & 15);
-          if (i < l && j < l) {
-            const i64 e = i + (i64)j * l, ll = (i64)l * l;
-            const T c = cld(a.Y[cur] + e), ci = cld(a.W[cur] + e);
-            const T d = c - ((i == j) ? T(1) : T(0));
-            sq = d * d;
-            T *w3 = a.W3 + l + i + (i64)j * 2 * l;
-            w3[0] = a.ninv * ci;
-            w3[2 * ll] = ci;
-            w3[4 * ll] = ci;
-          }
-        }
-        for (int off = 32; off > 0; off >>= 1) sq += __shfl_xor(sq, off, 64);
-        if ((threadIdx.x & 63) == 0) sh[1 + (threadIdx.x >> 6)] = sq;
-        __syncthreads();
-        if ((threadIdx.x & 255) == 0 && tile < ntile) {
-          const int w0 = 1 + (threadIdx.x >> 6);
-          cst(a.part + tile, sh[w0] + sh[w0 + 1] + sh[w0 + 2] + sh[w0 + 3]);
-        }
-        if (!chain_barrier(a.bar)) goto timeout;
-      }
       for (int i = t - 2; i >= 0; --i) {
         // Pf := Pf Z4 + Bf_i, Pg := Pg Z4 + Bg_i
         ChainProd<T> pf = prod(a.Y[cur], a.Z4, a.Y[cur ^ 1], false, T(1), T(0));
@@ -581,7 +575,7 @@ __device__ void chain_body(const ChainArgs<T> &a, typename SM<T>::acc (*red)[64]
         pf.epi = kEpiW3Up;
         round1<T>(a, pf, red);
       }
-      const T err2 = chain_sum<T>(a.part, ntile, sh);
+      const T err2 = chain_sum<T>(a.part + (t == 1 ? kPartErr : 0), ntile, sh);
       if (lead) {
         a.scal[0] = sqrt(err2);
         a.scal[3] =
