@@ -351,8 +351,8 @@ __device__ T chain_sum(const T *v, int n, T *sh) {
 
 // sum_j c[j] Z^j (j < 4) at entry e, Z^0 = I
 template <typename T>
-__device__ __forceinline__ T block_at(const ChainArgs<T> &a, const T *c, i64 e, T dl, T z3) {
-  return c[0] * dl + c[1] * cld(a.Z + e) + c[2] * cld(a.Z2 + e) + c[3] * z3;
+__device__ __forceinline__ T block_at(const T *c, T dl, T z, T z2, T z3) {
+  return c[0] * dl + c[1] * z + c[2] * z2 + c[3] * z3;
 }
 
 // One round over this workgroup's two tiles (2 wg, 2 wg + 1): product p0 and, with Dual, p1.
@@ -375,7 +375,27 @@ __device__ void chain_round(const ChainArgs<T> &a, const ChainProd<T> p0, const 
   const int lr = lane & 15, lk = lane >> 4;
   const int row = tm * 16 + lr, col = tn * 16 + lr;
   typename SM<T>::acc acc = {0, 0, 0, 0};
+  // the epilogue's other operands (C_in, Z, Z^2, Z^3 at this lane's four entries), requested
+  // with the product's operands: they were written in earlier rounds
+  const int epi = p.epi;
+  const bool need_z = epi & (kEpiStart1 | kEpiBlock | kEpiTop);
+  const bool need_z2 = epi & (kEpiBlock | kEpiTop);
+  const bool need_z3 = epi & kEpiBlock;
+  T pcin[4] = {0, 0, 0, 0}, pz[4] = {0, 0, 0, 0}, pz2[4] = {0, 0, 0, 0}, pz3[4] = {0, 0, 0, 0};
   if (live) {
+    if (q == 0) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        int r = tm * 16 + SM<T>::row(lane, g), c = tn * 16 + lr;
+        r = r < l ? r : l - 1;
+        c = c < l ? c : l - 1;
+        const i64 e = r + (i64)c * l;
+        if (p.Cin) pcin[g] = cld(p.Cin + e);
+        if (need_z) pz[g] = cld(a.Z + e);
+        if (need_z2) pz2[g] = cld(a.Z2 + e);
+        if (need_z3) pz3[g] = cld(a.Z3 + e);
+      }
+    }
     const int ksteps = (l + 3) / 4, spw = (ksteps + wpj - 1) / wpj;
     const int s0 = q * spw;
     const int rr = row < l ? row : l - 1, cc = col < l ? col : l - 1;
@@ -412,7 +432,6 @@ __device__ void chain_round(const ChainArgs<T> &a, const ChainProd<T> p0, const 
     }
     T sq = 0;
     const i64 ll = (i64)l * l;
-    const int epi = p.epi;
 #pragma unroll
     for (int g = 0; g < 4; ++g) {
       const int r = tm * 16 + SM<T>::row(lane, g), c = tn * 16 + lr;
@@ -420,18 +439,18 @@ __device__ void chain_round(const ChainArgs<T> &a, const ChainProd<T> p0, const 
         const i64 e = r + (i64)c * l;
         const T dl = (r == c) ? T(1) : T(0);
         T v = p.alpha * sum[g] + p.diag * dl;
-        if (p.Cin) v += p.cin_coef * cld(p.Cin + e);
-        if (epi & kEpiBlock) v += block_at(a, p.coef, e, dl, cld(a.Z3 + e));
+        if (p.Cin) v += p.cin_coef * pcin[g];
+        if (epi & kEpiBlock) v += block_at(p.coef, dl, pz[g], pz2[g], pz3[g]);
         if (p.out) cst(p.out + e, v);
         if (epi & kEpiStart1) {
-          const T z = cld(a.Z + e);
+          const T z = pz[g];
           cst(p.out2 + e, dl - T(2) * z - T(8) * v);
           cst(p.out3 + e, dl + T(2) * z);
         }
-        if (epi & kEpiStart2) cst(p.out2 + e, T(0.25) * (T(5) * dl - cld(p.Cin + e)));
+        if (epi & kEpiStart2) cst(p.out2 + e, T(0.25) * (T(5) * dl - pcin[g]));
         if (epi & kEpiTop) {
-          const T cv = block_at(a, p.coef, e, dl, v);
-          const T gv = block_at(a, p.coef + kSeriesTerms, e, dl, v);
+          const T cv = block_at(p.coef, dl, pz[g], pz2[g], v);
+          const T gv = block_at(p.coef + kSeriesTerms, dl, pz[g], pz2[g], v);
           cst(p.out2 + e, cv);
           cst(p.out3 + e, gv);
           if (epi & kEpiTopLast) {  // read only after the kernel: plain stores
