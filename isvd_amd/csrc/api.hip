@@ -192,6 +192,12 @@ void pinned_slot_record(int slot) {
   HIP_CHECK(hipEventRecord(g_slot_ev[slot], g_ctx.stream));
 }
 
+hipEvent_t pinned_slot_event(int slot) {
+  ISVD_CHECK(slot >= 0 && slot < kSlots, "pinned_slot_event: bad slot");
+  ensure_pinned();
+  return g_slot_ev[slot];
+}
+
 void read_back_wait(int slot, void *host, size_t bytes) {
   ISVD_CHECK(slot >= 0 && slot < kSlots && bytes <= kSlotBytes, "read_back_wait: bad slot");
   spin_on(g_slot_ev[slot]);

@@ -151,6 +151,7 @@ void read_back(void *host, const void *dev, size_t bytes);
 // the bytes out.  A slot holds one pending value at a time.
 void *pinned_slot(int slot);
 void pinned_slot_record(int slot);
+hipEvent_t pinned_slot_event(int slot);  // the event pinned_slot_record records
 void read_back_wait(int slot, void *host, size_t bytes);
 
 // True when ptr is device (HBM) memory usable in place by kernels on the current device.

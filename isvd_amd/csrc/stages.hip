@@ -414,7 +414,7 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
   // Its workgroups fill a CU's registers each: it leaves the chain's CUs free, or the chain,
   // launched beside it, would wait for G's workgroups to finish before its own could start.
   auto g_product = [&](i64 q) {
-    HIP_CHECK(hipStreamWaitEvent(ctx().aux, ctx().fork, 0));
+    HIP_CHECK(hipStreamWaitEvent(ctx().aux, pinned_slot_event((int)(q & 1)), 0));
     {
       OnStream on(ctx().aux);
       GridCap cap(ctx().cus - kn_chain_blocks(l));
@@ -424,25 +424,26 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
   };
   // after the join, one batched launch: X_+ := X_c F + (B_c / N) C^-1 (image of :221-222),
   // B_+ := B_c F + Bg_c C^-1 (:225-226), Bg_+ := Bg_c F + G_c C^-1 ([X | B], [B | Bg] and
-  // [Bg | G] are consecutive column blocks of the state); the fork for the next G; the next
-  // Gram.  (Updating X and B before the join would run them beside G on the few CUs G leaves
-  // to the chain.)
+  // [Bg | G] are consecutive column blocks of the state); then the event of pinned slot
+  // (q + 1) % 2, which is both the fork for the next G (state q + 1 is written) and the host's
+  // signal that the chain of iteration q + 1 has stored its scalars in that slot (one marker on
+  // the library stream per iteration); the next Gram.  (Updating X and B before the join would
+  // run them beside G on the few CUs G leaves to the chain.)
   auto updates = [&](i64 q) {
     HIP_CHECK(hipStreamWaitEvent(ctx().stream, ctx().join, 0));
     gemm_shortk<T>('N', 'N', Nl, l, 2 * l, T(1), X(q), Nl, w3.get(), 2 * l, T(0), X(q + 1), Nl,
                    3, Nl * l, 2 * ll, Nl * l);
-    HIP_CHECK(hipEventRecord(ctx().fork, ctx().stream));
+    pinned_slot_record((int)((q + 1) & 1));
     gram(q + 1);
   };
   // iteration it from state it - 1 (queued; the chain stores err, rho and its status straight
-  // into pinned slot it % 2, ready for the host as soon as the chain is done)
+  // into pinned slot it % 2, whose event the updates behind it record)
   T *slot[2] = {static_cast<T *>(pinned_slot(0)), static_cast<T *>(pinned_slot(1))};
   auto enqueue = [&](i64 it, int m) {
     const i64 q = it - 1;
     g_product(q);
     // [D | Dg]; C and C^-1 (:193-212): C = (I/2 + (I/4 - Z)^{1/2})^{1/2}; F (:218); W3
     kn_chain<T>((int)l, ninv, parts.get(), nparts, ddg.get(), chain, w3.get(), slot[it & 1], m);
-    pinned_slot_record((int)(it & 1));
     updates(q);
   };
   // X_c := [I; 0] (Qc := Q_0, :156);  B_c := Qs^T Q_0 = M[:, 0:l]  (:159-160);
@@ -455,7 +456,7 @@ void integrate_kn_coords(const Dims &d, const KnArgs &args, const T *M, T *x_out
     omatcopy<T>('N', Nl, l, T(1), M, Nl, B(0), Nl);
   }
   m_apply(B(0), ninv, Bg(0));
-  HIP_CHECK(hipEventRecord(ctx().fork, ctx().stream));
+  pinned_slot_record(0);  // the fork for G_0
   gram(0);
   enqueue(1, mode);
   i64 it;
