@@ -10,6 +10,8 @@
 // A sweep with no rotation above the relative threshold ends the iteration.
 #include "la.hpp"
 
+#include <type_traits>
+
 #include "dpp.hpp"
 
 #include <cfloat>
@@ -159,8 +161,18 @@ __global__ __launch_bounds__(kMaxThreads) void svd_jacobi_kernel(
         gamma = lane_group_sum(gamma, TPP);
         // |gamma| > tol sqrt(alpha beta), compared in squares
         if (gamma != T(0) && gamma * gamma > tol * tol * (alpha * beta)) {
-          const T zeta = (beta - alpha) / (T(2) * gamma);
-          const T t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
+          // t = sign(zeta) / (|zeta| + sqrt(1 + zeta^2)), zeta = (beta - alpha) / (2 gamma); in
+          // FP64 multiplied through by |2 gamma| (one division less on the round's dependent
+          // chain; the squares stay far from overflow at FP64 range), sign(zeta) = +1 at zeta = 0
+          T t;
+          if constexpr (std::is_same_v<T, double>) {
+            const T d = beta - alpha, g2 = T(2) * gamma;
+            const bool pos = d == T(0) || ((d > T(0)) == (g2 > T(0)));
+            t = (pos ? fabs(g2) : -fabs(g2)) / (fabs(d) + sqrt(d * d + g2 * g2));
+          } else {
+            const T zeta = (beta - alpha) / (T(2) * gamma);
+            t = (zeta >= T(0) ? T(1) : T(-1)) / (fabs(zeta) + sqrt(T(1) + zeta * zeta));
+          }
           const T c = T(1) / sqrt(T(1) + t * t);
           const T s = c * t;
 #pragma unroll
